@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark: LP solves/sec of the restarted PDHG solver on one MI355X, with the SpMV-kernel roofline.
+
+Workload (default): the single large planted LP — m = 1,000,000 equality rows, n = 2,000,000 columns,
+20 nonzeros per column at uniform-random rows, N(0,1) values, 0 <= x <= 10, seed 1 — solved to 1e-6
+relative KKT (original-space primal residual, dual residual and gap, the reference's stopping test).
+A "step" is one full solve from a cold start of a problem already resident in HBM (power iteration,
+restarted PDHG iterations, FP64 evaluations, download and the original-space check).
+
+Printed JSON line: the contract's fields, plus
+  roofline: the dominant iteration kernel (A^T z fused with the primal step, or A x fused with the dual
+            step): algorithmic bytes per launch / mean launch time from HIP events on the solver stream
+            during the timed solves, against the 8 TB/s HBM peak;
+  cpu_baseline: the CPU oracle (one thread, FP64) timed on a bounded sample of the same workload (a few
+            PDHG iterations), scaled to solves/sec with the GPU solve's iteration count;
+  fp64_solves_per_s / speedup_vs_fp64 when the timed mode is not FP64 (plain FP64 cold start is timed too).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "taral-lp_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "LP solves/sec to 1e-6 rel KKT (learned+FP32 vs plain FP64 PDHG); SpMV HBM GB/s"
+
+WORKLOADS = {  # name: (m, n, nnz per column, seed)
+    "cfg1": (1_000_000, 2_000_000, 20, 1),
+    "cfg0": (2_000, 5_000, 10, 0),
+    "mid": (100_000, 200_000, 20, 1),
+}
+
+
+def kernel_bytes(m, n, nnz, s):
+    """Algorithmic HBM bytes of one launch of each fused iteration kernel (s = bytes per value):
+    row kernel: val+idx (s+4)/nnz, row pointers 4(m+1), x read once s*n, and per row z, Ax, rl, ru, Axs, zs
+    read and Ax, z, Axs, zs written (10 s per row); column kernel likewise over A^T with 10 s per column."""
+    row = (s + 4) * nnz + 4 * (m + 1) + s * n + 10 * s * m
+    col = (s + 4) * nnz + 4 * (n + 1) + s * m + 10 * s * n
+    return row, col
+
+
+def mode_options(mode):
+    if mode == "fp64":
+        return dict(precision_bits=64)
+    if mode == "fp32":
+        return dict(precision_bits=32)
+    if mode == "fp32-refine":
+        return dict(precision_bits=32, refine=1)
+    raise SystemExit(f"unknown mode {mode}")
+
+
+def cpu_baseline(md, iterations, sample_iters):
+    """Oracle FP64 (one thread): time `sample_iters` PDHG iterations of the same problem, scale to one solve
+    of `iterations` iterations (power-iteration steps counted as iterations too)."""
+    from oracle import oracle
+    o = oracle.OracleLP(md)
+    t0 = time.perf_counter()
+    o.block(0.5, 0.5, 0)  # start: two products
+    t_start = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.block(0.5, 0.5, sample_iters)
+    t = time.perf_counter() - t0 - t_start
+    per_iter = max(t, 1e-9) / sample_iters
+    return dict(value=1.0 / (per_iter * iterations), unit="solves/s", cores=1, kind="port",
+                sample=f"CPU oracle (oracle/pdhg_oracle.c, FP64, 1 thread): {sample_iters} PDHG iterations of the "
+                       f"same LP took {t:.3f}s ({per_iter * 1e3:.1f} ms/iter); scaled to the GPU solve's "
+                       f"{iterations} iterations", ms_per_iter=per_iter * 1e3)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg1", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="auto", choices=["auto", "fp64", "fp32", "fp32-refine"])
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--time-limit", type=float, default=600.0)
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--cpu-sample-iters", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU sharding is a design only "
+              "(see the sharding section of the design notes)", file=sys.stderr)
+        sys.exit(2)
+    mode = "fp64" if a.mode == "auto" else a.mode
+
+    from taral_pdhg import ffi, gen
+    m, n, k, seed = WORKLOADS[a.workload]
+    t = time.perf_counter()
+    md = gen.planted_lp(m, n, k, seed=seed)
+    model = ffi.Model.from_lp(md)
+    print(f"[bench] generated {m}x{n} nnz={md.nnz} in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+
+    def timed(mode_name, steps, warmup, profile):
+        opts = dict(tol=a.tol, time_limit=a.time_limit, profile_kernels=int(profile), **mode_options(mode_name))
+        t0 = time.perf_counter()
+        sess = ffi.Session(model, **opts)
+        setup = time.perf_counter() - t0
+        print(f"[bench] {mode_name}: session setup {setup:.1f}s", file=sys.stderr, flush=True)
+        for w in range(warmup):
+            r = sess.solve()
+            print(f"[bench] {mode_name} warmup {w}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s",
+                  file=sys.stderr, flush=True)
+        results = []
+        t0 = time.perf_counter()
+        for s in range(steps):
+            r = sess.solve()
+            results.append(r)
+            print(f"[bench] {mode_name} step {s}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s",
+                  file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        sess.close()
+        return wall, results, setup
+
+    wall, res, setup = timed(mode, a.steps, a.warmup, profile=True)
+    last = res[-1]
+    value = a.steps / wall
+    s_val = 8 if mode == "fp64" else 4
+    b_row, b_col = kernel_bytes(m, n, last["nnz"], s_val)
+    row_ms = sum(r["row_kernel_ms"] for r in res)
+    col_ms = sum(r["col_kernel_ms"] for r in res)
+    row_n = sum(r["row_launches"] for r in res)
+    col_n = sum(r["col_launches"] for r in res)
+    row_avg = row_ms / max(row_n, 1)
+    col_avg = col_ms / max(col_n, 1)
+    if col_ms >= row_ms:
+        dom, dom_bytes, dom_avg = "col (A^T z + primal step)", b_col, col_avg
+    else:
+        dom, dom_bytes, dom_avg = "row (A x + dual step)", b_row, row_avg
+    achieved = dom_bytes / (dom_avg * 1e-3) / 1e9 if dom_avg > 0 else 0.0
+    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=dom_bytes,
+                    avg_launch_ms=round(dom_avg, 5), row_kernel_avg_ms=round(row_avg, 5),
+                    col_kernel_avg_ms=round(col_avg, 5),
+                    row_gbs=round(b_row / (row_avg * 1e-3) / 1e9, 1) if row_avg else 0,
+                    col_gbs=round(b_col / (col_avg * 1e-3) / 1e9, 1) if col_avg else 0)
+    p_star = md.planted_objective
+    out = {
+        "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 (f64 reductions)", "fp32-refine": "f32 iterates + f64 refinement"}[mode],
+        "data": "synthetic",
+        "config": {"workload": f"planted equality LP m={m} n={n} nnz/col={k} seed={seed}, 0<=x<=10, tol {a.tol:g}",
+                   "m": m, "n": n, "nnz": last["nnz"], "tol": a.tol, "mode": mode, "parallelism": "single GPU"},
+        "iterations": last["iterations"], "restarts": last["restarts"], "status": last["status"],
+        "rel_kkt": max(last["rel_primal"], last["rel_dual"], last["rel_gap"]),
+        "objective_rel_error_vs_planted": abs(last["objective"] - p_star) / (1 + abs(p_star)),
+        "iters_per_s": round(last["iterations"] / last["solve_s"], 1), "session_setup_s": round(setup, 3),
+        "device": last["device"], "roofline": roofline,
+    }
+    if mode != "fp64" and not a.no_fp64_compare:
+        w64, r64, _ = timed("fp64", 1, 0, profile=False)
+        out["fp64_solves_per_s"] = round(1 / w64, 6)
+        out["fp64_iterations"] = r64[-1]["iterations"]
+        out["speedup_vs_fp64"] = round(value * w64, 4)
+    if not a.no_cpu_baseline:
+        print("[bench] cpu baseline sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(md, last["iterations"], a.cpu_sample_iters)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,186 @@
+/* taral_pdhg.h — C ABI of the MI355X restarted-PDHG LP solver (libtaral_pdhg.so).
+ *
+ * Drop-in boundary for the reference's PDHG path (Hexraei/TARAL-LP, gpu/pdhg.cu + gpu/pdhg_input.hpp over the
+ * Model/read_mps contract of src/taral.hpp and src/mps.cpp). Plain pointers and sizes only; every call
+ * returns TPDHG_OK (0) or a negative code, with the message in tpdhg_last_error() (per thread). Nothing here
+ * calls exit(): the reference's CHECK macro (gpu/pdhg.cu:36-43) exits the process, this boundary reports.
+ * There is no CPU compute path: without a HIP device the device and solve calls fail with
+ * TPDHG_ERR_NO_DEVICE.
+ *
+ * Handles are not thread-safe; one device handle owns one HIP stream and its device memory.
+ */
+#ifndef TARAL_PDHG_H
+#define TARAL_PDHG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TPDHG_OK = 0,
+    TPDHG_ERR_ARG = -1,          /* bad argument or dimensions */
+    TPDHG_ERR_PARSE = -2,        /* MPS parse error (reference: ParseError, pdhg.cu:977-980, exit 1) */
+    TPDHG_ERR_UNSUPPORTED = -3,  /* QP or MILP input (reference: pdhg_reject_unsupported_input, exit 2) */
+    TPDHG_ERR_DEVICE = -4,       /* HIP runtime failure (reference: CHECK -> exit 1) */
+    TPDHG_ERR_NO_DEVICE = -5,    /* no HIP device visible */
+    TPDHG_ERR_TIME = -6          /* parse deadline passed (reference: ParseTimeLimit) */
+};
+
+/* Solve status, the strings of gpu/pdhg.cu:753,814-826. */
+enum { TPDHG_NEAR_OPTIMAL = 0, TPDHG_ITERATION_LIMIT = 1, TPDHG_TIME_LIMIT = 2, TPDHG_NUMERICAL_FAILURE = 3 };
+
+const char* tpdhg_last_error(void);
+const char* tpdhg_status_string(int status);
+int tpdhg_device_count(void);
+
+/* ------------------------------------------------------------------ model (input contract) */
+typedef struct tpdhg_model tpdhg_model;
+
+/* read_mps(path) (src/taral.hpp:48, src/mps.cpp:388-399): free format, then fixed columns. */
+int tpdhg_model_read_mps(const char* path, tpdhg_model** out);
+/* Same reader over MPS text in memory (len bytes). */
+int tpdhg_model_read_mps_text(const char* text, int64_t len, tpdhg_model** out);
+/* Column-wise LP from arrays (CSC; rows ascending within each column), the fields of Model
+ * (src/taral.hpp:24-35) that the PDHG path reads. Bounds may be +-INFINITY. Arrays are copied. */
+int tpdhg_model_from_csc(int32_t m, int32_t n, const int64_t* colptr, const int32_t* rowidx, const double* val,
+                         const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                         const double* row_up, double obj_const, int32_t maximize, tpdhg_model** out);
+void tpdhg_model_free(tpdhg_model* model);
+
+typedef struct {
+    int32_t m, n;
+    int64_t nnz;
+    int32_t maximize;
+    int32_t has_integers;
+    int64_t qobj_terms;
+    int64_t sentinel_bounds;
+    double obj_const;
+} tpdhg_model_info;
+int tpdhg_model_get_info(const tpdhg_model* model, tpdhg_model_info* info);
+/* Copies the model out; any pointer may be NULL. Sizes: colptr n+1, rowidx/val nnz, cost/col_lo/col_up/is_int n,
+ * row_lo/row_up m. */
+int tpdhg_model_export(const tpdhg_model* model, int64_t* colptr, int32_t* rowidx, double* val, double* cost,
+                       double* col_lo, double* col_up, double* row_lo, double* row_up, int8_t* is_int);
+/* Name of row i (is_row != 0) or column i; "" for array-built models. */
+const char* tpdhg_model_name_of(const tpdhg_model* model, int32_t is_row, int32_t i);
+/* pdhg_input_error (gpu/pdhg_input.hpp:7-17): the reason PDHG refuses this model, or NULL. */
+const char* tpdhg_model_input_error(const tpdhg_model* model);
+
+/* ------------------------------------------------------------------ scaled problem (host, FP64) */
+typedef struct tpdhg_scaled tpdhg_scaled;
+typedef struct {
+    int32_t m, n;
+    int64_t nnz;
+    const int32_t *a_ptr, *a_idx; /* A_s, CSR over m rows */
+    const double* a_val;
+    const int32_t *at_ptr, *at_idx; /* A_s^T, CSR over n rows */
+    const double* at_val;
+    const double *c, *l, *u, *rl, *ru, *Dr, *Dc;
+    double bnorm, cnorm;
+} tpdhg_scaled_arrays;
+
+/* build_scaled(model, sign) (gpu/pdhg.cu:80-133): Ruiz x10 (inf-norm) then Pock-Chambolle (alpha 1). */
+int tpdhg_scaled_build(const tpdhg_model* model, tpdhg_scaled** out);
+/* Pointers into the handle's memory (valid until tpdhg_scaled_free). */
+int tpdhg_scaled_view(const tpdhg_scaled* s, tpdhg_scaled_arrays* view);
+void tpdhg_scaled_free(tpdhg_scaled* s);
+
+/* ------------------------------------------------------------------ device backend (the Gpu<T> concept) */
+typedef struct tpdhg_dev tpdhg_dev;
+
+/* Gpu<T>::init (gpu/pdhg.cu:362-382): uploads A_s, A_s^T and the vectors; precision_bits 64 (FP64 iterates) or
+ * 32 (FP32 iterates and SpMV, FP64 reductions; the reference's --fp32). */
+int tpdhg_dev_create(const tpdhg_scaled_arrays* problem, int32_t precision_bits, int32_t eval_freq,
+                     int32_t use_graph, tpdhg_dev** out);
+/* Gpu<T>::power_norm (gpu/pdhg.cu:417-436). */
+int tpdhg_dev_power_norm(tpdhg_dev* dev, int32_t iters, double* norm);
+/* Gpu<T>::set_steps (gpu/pdhg.cu:442-445). */
+int tpdhg_dev_set_steps(tpdhg_dev* dev, double tau, double sigma);
+/* Gpu<T>::start (gpu/pdhg.cu:450-458); x0/z0 (scaled space) may be NULL for the reference cold start. */
+int tpdhg_dev_start(tpdhg_dev* dev, const double* x0, const double* z0);
+/* Gpu<T>::iterate (gpu/pdhg.cu:471-486). */
+int tpdhg_dev_iterate(tpdhg_dev* dev, int32_t steps);
+/* Gpu<T>::eval (gpu/pdhg.cu:487-499): out[0..4] = PR2S, PR2O, PR2R, DROW, DZ2; out[5..9] = DR2S, DR2O, POBJ,
+ * DCOL, DX2. */
+int tpdhg_dev_eval(tpdhg_dev* dev, int32_t avg, double f, double out[10]);
+/* Current and f-scaled average KKT in one fused pass (no reference counterpart; same values as two evals). */
+int tpdhg_dev_eval2(tpdhg_dev* dev, double f, double cur[10], double avg[10]);
+/* Gpu<T>::restart (gpu/pdhg.cu:500-504). */
+int tpdhg_dev_restart(tpdhg_dev* dev, int32_t avg, double f);
+/* Gpu<T>::download (gpu/pdhg.cu:505-513): x (n) and z (m) in FP64, scaled space; either may be NULL. */
+int tpdhg_dev_download(tpdhg_dev* dev, int32_t avg, double f, double* x, double* z);
+/* y = A_s v (transpose 0; v n, y m) or A_s^T v (transpose 1; v m, y n) with the solver's SpMV kernel. */
+int tpdhg_dev_spmv(tpdhg_dev* dev, int32_t transpose, const double* v, double* y);
+/* Gpu<T>::mem_used and name (gpu/pdhg.cu:331-332); name_buf receives up to name_len-1 chars. */
+int tpdhg_dev_info(tpdhg_dev* dev, int64_t* mem_used, char* name_buf, int32_t name_len);
+/* Per-kernel HIP-event timing of the iteration kernels (on the solver stream) from now on. */
+int tpdhg_dev_set_profile(tpdhg_dev* dev, int32_t on);
+int tpdhg_dev_kernel_times(tpdhg_dev* dev, double* row_ms, double* col_ms, int64_t* row_launches,
+                           int64_t* col_launches);
+void tpdhg_dev_destroy(tpdhg_dev* dev);
+
+/* ------------------------------------------------------------------ full solve (the pdhg CLI path) */
+typedef struct {
+    double tol;        /* relative KKT tolerance (default 1e-4, gpu/pdhg.cu:745) */
+    double time_limit; /* seconds (default 3600) */
+    int64_t max_iter;  /* default 2e9 */
+    int32_t eval_freq; /* iterations per evaluation (default 64) */
+    double row_rel;    /* per-row relative primal test factor, 0 = off (gpu/pdhg.cu:748) */
+    int32_t precision_bits; /* 64 (default) or 32 */
+    int32_t refine;    /* 1: FP64 iterative refinement around FP32 inner solves (precision_bits 32) */
+    double inner_tol;  /* relative KKT target of each refinement round (default 1e-4) */
+    int32_t max_refinements;
+    int32_t use_graph; /* capture each eval_freq-iteration block in a HIP graph (default 1) */
+    int32_t verbose;
+    int32_t profile_kernels; /* HIP-event timing of the row/column kernels */
+} tpdhg_options;
+void tpdhg_options_default(tpdhg_options* o);
+
+typedef struct {
+    int32_t status; /* TPDHG_NEAR_OPTIMAL ... */
+    int64_t iterations;
+    int32_t restarts, refinements, from_average;
+    double objective; /* original model, long double accumulation (gpu/pdhg.cu:868-869) */
+    double rel_primal, rel_dual, rel_gap;
+    double max_row_violation, max_bound_violation;
+    double setup_s, parse_s, scale_s, device_init_s, transfer_s, solve_s, power_iter_s, total_s;
+    double norm_a, primal_weight;
+    int64_t device_mem_bytes;
+    int32_t m, n;
+    int64_t nnz;
+    double row_kernel_ms, col_kernel_ms; /* HIP-event totals when profile_kernels */
+    int64_t row_launches, col_launches;
+    char device[64];
+} tpdhg_result;
+
+/* Scales, uploads and solves (gpu/pdhg.cu:913-935 -> finish -> run). x (n) receives the original-space point
+ * and y (m) the row multipliers y = Dr z (gpu/pdhg.cu:905-909); either may be NULL. x0/y0 (same spaces,
+ * nullable) warm start the solve. Refuses QP/MILP models with TPDHG_ERR_UNSUPPORTED before any device work. */
+int tpdhg_solve(const tpdhg_model* model, const tpdhg_options* opt, const double* x0, const double* y0,
+                tpdhg_result* result, double* x, double* y);
+/* A problem kept resident on the device: scaled and uploaded once by tpdhg_session_create, then solved from a
+ * cold (x0 = y0 = NULL) or warm start by each tpdhg_session_solve, which repeats the power iteration and the
+ * PDHG loop (no reference counterpart: the reference CLI re-reads and re-uploads per solve). The session keeps
+ * a copy of the model. */
+typedef struct tpdhg_session tpdhg_session;
+int tpdhg_session_create(const tpdhg_model* model, const tpdhg_options* opt, tpdhg_session** out);
+int tpdhg_session_solve(tpdhg_session* s, const double* x0, const double* y0, tpdhg_result* result, double* x,
+                        double* y);
+void tpdhg_session_free(tpdhg_session* s);
+
+/* The JSON object gpu/pdhg.cu:881-897 writes (same keys and formats, plus this build's keys). */
+int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const tpdhg_result* result, char* buf,
+                      int32_t len);
+
+/* ------------------------------------------------------------------ SpMV microbenchmark (gpu/csr_matvec.cu) */
+/* y = M v for a CSR (rows x cols) on the device with the solver's SpMV kernel; `reps` timed launches after a
+ * warm-up, mean and median kernel time from HIP events. */
+int tpdhg_spmv_bench(int32_t precision_bits, int32_t rows, int32_t cols, const int32_t* ptr, const int32_t* idx,
+                     const double* val, const double* v, double* y, int32_t reps, double* mean_ms,
+                     double* median_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,367 @@
+// extern "C" boundary of libtaral_pdhg.so (declared in include/taral_pdhg.h): handles, return codes and a
+// per-thread error message around the C++ host code and the HIP device backend.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/taral_pdhg.h"
+#include "driver.hpp"
+
+struct tpdhg_model {
+    tpdhg::LP lp;
+};
+struct tpdhg_scaled {
+    tpdhg::Scaled s;
+};
+struct tpdhg_dev {
+    std::unique_ptr<tpdhg::Device> d;
+};
+struct tpdhg_session {
+    tpdhg::LP lp;  // the session's own copy; Session keeps a reference to it
+    std::unique_ptr<tpdhg::Session> s;
+};
+
+namespace {
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        g_error.clear();
+        return f();
+    } catch (const tpdhg::ParseError& e) {
+        return fail(TPDHG_ERR_PARSE, e.what());
+    } catch (const tpdhg::ParseTimeLimit& e) {
+        return fail(TPDHG_ERR_TIME, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(TPDHG_ERR_ARG, "out of host memory");
+    } catch (const std::runtime_error& e) {
+        const std::string w = e.what();
+        if (w.find("no HIP device") != std::string::npos) return fail(TPDHG_ERR_NO_DEVICE, w);
+        return fail(TPDHG_ERR_DEVICE, w);
+    } catch (const std::exception& e) {
+        return fail(TPDHG_ERR_ARG, e.what());
+    }
+}
+
+int status_code(const std::string& s) {
+    if (s == "near_optimal") return TPDHG_NEAR_OPTIMAL;
+    if (s == "time_limit") return TPDHG_TIME_LIMIT;
+    if (s == "numerical_failure") return TPDHG_NUMERICAL_FAILURE;
+    return TPDHG_ITERATION_LIMIT;
+}
+
+tpdhg::Options to_options(const tpdhg_options* o) {
+    tpdhg::Options r;
+    if (!o) return r;
+    r.tol = o->tol, r.time_limit = o->time_limit, r.max_iter = o->max_iter, r.eval_freq = o->eval_freq;
+    r.row_rel = o->row_rel, r.precision_bits = o->precision_bits, r.refine = o->refine != 0;
+    r.inner_tol = o->inner_tol, r.max_refinements = o->max_refinements, r.graph = o->use_graph != 0;
+    r.verbose = o->verbose != 0, r.profile_kernels = o->profile_kernels != 0;
+    return r;
+}
+}  // namespace
+
+extern "C" {
+
+const char* tpdhg_last_error(void) { return g_error.c_str(); }
+
+const char* tpdhg_status_string(int status) {
+    switch (status) {
+        case TPDHG_NEAR_OPTIMAL: return "near_optimal";
+        case TPDHG_TIME_LIMIT: return "time_limit";
+        case TPDHG_NUMERICAL_FAILURE: return "numerical_failure";
+        default: return "iteration_limit";
+    }
+}
+
+int tpdhg_device_count(void) { return tpdhg::device_count(); }
+
+int tpdhg_model_read_mps(const char* path, tpdhg_model** out) {
+    if (!path || !out) return fail(TPDHG_ERR_ARG, "null argument");
+    return guarded([&]() -> int {
+        auto m = std::make_unique<tpdhg_model>();
+        m->lp = tpdhg::flatten(tpdhg::read_mps(path));
+        *out = m.release();
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_model_read_mps_text(const char* text, int64_t len, tpdhg_model** out) {
+    if (!text || !out || len < 0) return fail(TPDHG_ERR_ARG, "null argument");
+    return guarded([&]() -> int {
+        auto m = std::make_unique<tpdhg_model>();
+        m->lp = tpdhg::flatten(tpdhg::read_mps_text(std::string(text, size_t(len))));
+        *out = m.release();
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_model_from_csc(int32_t m, int32_t n, const int64_t* colptr, const int32_t* rowidx, const double* val,
+                         const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                         const double* row_up, double obj_const, int32_t maximize, tpdhg_model** out) {
+    if (m < 0 || n < 0 || !colptr || !out || (n > 0 && (!cost || !col_lo || !col_up)) ||
+        (m > 0 && (!row_lo || !row_up)))
+        return fail(TPDHG_ERR_ARG, "bad model arrays");
+    return guarded([&]() -> int {
+        const int64_t nnz = colptr[n];
+        if (colptr[0] != 0 || nnz < 0 || nnz > INT32_MAX || (nnz > 0 && (!rowidx || !val)))
+            return fail(TPDHG_ERR_ARG, "bad column pointers");
+        for (int j = 0; j < n; ++j) {
+            if (colptr[j + 1] < colptr[j]) return fail(TPDHG_ERR_ARG, "column pointers decrease");
+            for (int64_t k = colptr[j]; k < colptr[j + 1]; ++k) {
+                if (rowidx[k] < 0 || rowidx[k] >= m) return fail(TPDHG_ERR_ARG, "row index out of range");
+                if (k > colptr[j] && rowidx[k] <= rowidx[k - 1])
+                    return fail(TPDHG_ERR_ARG, "row indices must ascend strictly within a column");
+                if (!std::isfinite(val[k])) return fail(TPDHG_ERR_ARG, "matrix value not finite");
+            }
+        }
+        auto md = std::make_unique<tpdhg_model>();
+        tpdhg::LP& lp = md->lp;
+        lp.name = "LP", lp.m = m, lp.n = n;
+        lp.colptr.assign(colptr, colptr + n + 1);
+        lp.rowidx.assign(rowidx, rowidx + nnz), lp.val.assign(val, val + nnz);
+        lp.cost.assign(cost, cost + n), lp.col_lo.assign(col_lo, col_lo + n), lp.col_up.assign(col_up, col_up + n);
+        lp.row_lo.assign(row_lo, row_lo + m), lp.row_up.assign(row_up, row_up + m);
+        lp.is_int.assign(size_t(n), 0);
+        lp.obj_const = obj_const, lp.maximize = maximize != 0;
+        *out = md.release();
+        return TPDHG_OK;
+    });
+}
+
+void tpdhg_model_free(tpdhg_model* model) { delete model; }
+
+int tpdhg_model_get_info(const tpdhg_model* model, tpdhg_model_info* info) {
+    if (!model || !info) return fail(TPDHG_ERR_ARG, "null argument");
+    const tpdhg::LP& lp = model->lp;
+    info->m = lp.m, info->n = lp.n, info->nnz = lp.nnz(), info->maximize = lp.maximize;
+    info->has_integers = 0;
+    for (char c : lp.is_int) info->has_integers |= c != 0;
+    info->qobj_terms = int64_t(lp.qobj_terms), info->sentinel_bounds = int64_t(lp.sentinel_bounds);
+    info->obj_const = lp.obj_const;
+    return TPDHG_OK;
+}
+
+int tpdhg_model_export(const tpdhg_model* model, int64_t* colptr, int32_t* rowidx, double* val, double* cost,
+                       double* col_lo, double* col_up, double* row_lo, double* row_up, int8_t* is_int) {
+    if (!model) return fail(TPDHG_ERR_ARG, "null model");
+    const tpdhg::LP& lp = model->lp;
+    auto cp = [](auto* dst, const auto& src) {
+        if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    cp(colptr, lp.colptr), cp(rowidx, lp.rowidx), cp(val, lp.val), cp(cost, lp.cost), cp(col_lo, lp.col_lo);
+    cp(col_up, lp.col_up), cp(row_lo, lp.row_lo), cp(row_up, lp.row_up);
+    if (is_int)
+        for (size_t j = 0; j < lp.is_int.size(); ++j) is_int[j] = int8_t(lp.is_int[j]);
+    return TPDHG_OK;
+}
+
+const char* tpdhg_model_name_of(const tpdhg_model* model, int32_t is_row, int32_t i) {
+    if (!model || i < 0) return "";
+    const auto& names = is_row ? model->lp.row_names : model->lp.col_names;
+    return size_t(i) < names.size() ? names[size_t(i)].c_str() : "";
+}
+
+const char* tpdhg_model_input_error(const tpdhg_model* model) { return model ? tpdhg::input_error(model->lp) : nullptr; }
+
+int tpdhg_scaled_build(const tpdhg_model* model, tpdhg_scaled** out) {
+    if (!model || !out) return fail(TPDHG_ERR_ARG, "null argument");
+    return guarded([&]() -> int {
+        const tpdhg::LP& lp = model->lp;
+        auto s = std::make_unique<tpdhg_scaled>();
+        s->s = tpdhg::build_scaled_csc(lp.m, lp.n, lp.colptr.data(), lp.rowidx.data(), lp.val.data(), lp.cost.data(),
+                                       lp.col_lo.data(), lp.col_up.data(), lp.row_lo.data(), lp.row_up.data(),
+                                       lp.maximize ? -1.0 : 1.0);
+        *out = s.release();
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_scaled_view(const tpdhg_scaled* s, tpdhg_scaled_arrays* v) {
+    if (!s || !v) return fail(TPDHG_ERR_ARG, "null argument");
+    const tpdhg::Scaled& P = s->s;
+    v->m = P.m, v->n = P.n, v->nnz = P.A.nnz();
+    v->a_ptr = P.A.ptr.data(), v->a_idx = P.A.idx.data(), v->a_val = P.A.val.data();
+    v->at_ptr = P.AT.ptr.data(), v->at_idx = P.AT.idx.data(), v->at_val = P.AT.val.data();
+    v->c = P.c.data(), v->l = P.l.data(), v->u = P.u.data(), v->rl = P.rl.data(), v->ru = P.ru.data();
+    v->Dr = P.Dr.data(), v->Dc = P.Dc.data(), v->bnorm = P.bnorm, v->cnorm = P.cnorm;
+    return TPDHG_OK;
+}
+
+void tpdhg_scaled_free(tpdhg_scaled* s) { delete s; }
+
+int tpdhg_dev_create(const tpdhg_scaled_arrays* p, int32_t precision_bits, int32_t eval_freq, int32_t use_graph,
+                     tpdhg_dev** out) {
+    if (!p || !out || p->m < 0 || p->n < 0 || !p->a_ptr || !p->at_ptr) return fail(TPDHG_ERR_ARG, "bad problem");
+    return guarded([&]() -> int {
+        tpdhg::DeviceOptions o;
+        o.precision_bits = precision_bits, o.eval_freq = eval_freq, o.use_graph = use_graph != 0;
+        auto d = std::make_unique<tpdhg_dev>();
+        d->d = tpdhg::make_device_raw(p->m, p->n, p->nnz, p->a_ptr, p->a_idx, p->a_val, p->at_ptr, p->at_idx,
+                                      p->at_val, p->c, p->l, p->u, p->rl, p->ru, p->Dr, p->Dc, o);
+        *out = d.release();
+        return TPDHG_OK;
+    });
+}
+
+#define DEV_CALL(dev, body)                                           \
+    do {                                                              \
+        if (!(dev) || !(dev)->d) return fail(TPDHG_ERR_ARG, "null device"); \
+        return guarded([&]() -> int { body; return TPDHG_OK; });               \
+    } while (0)
+
+int tpdhg_dev_power_norm(tpdhg_dev* dev, int32_t iters, double* norm) { DEV_CALL(dev, *norm = dev->d->power_norm(iters)); }
+int tpdhg_dev_set_steps(tpdhg_dev* dev, double tau, double sigma) { DEV_CALL(dev, dev->d->set_steps(tau, sigma)); }
+int tpdhg_dev_start(tpdhg_dev* dev, const double* x0, const double* z0) { DEV_CALL(dev, dev->d->start(x0, z0)); }
+int tpdhg_dev_iterate(tpdhg_dev* dev, int32_t steps) { DEV_CALL(dev, dev->d->iterate(steps)); }
+int tpdhg_dev_eval(tpdhg_dev* dev, int32_t avg, double f, double out[10]) {
+    DEV_CALL(dev, if (avg) dev->d->eval2(f, nullptr, out); else dev->d->eval2(f, out, nullptr));
+}
+int tpdhg_dev_eval2(tpdhg_dev* dev, double f, double cur[10], double avg[10]) { DEV_CALL(dev, dev->d->eval2(f, cur, avg)); }
+int tpdhg_dev_restart(tpdhg_dev* dev, int32_t avg, double f) { DEV_CALL(dev, dev->d->restart(avg != 0, f)); }
+int tpdhg_dev_download(tpdhg_dev* dev, int32_t avg, double f, double* x, double* z) {
+    DEV_CALL(dev, dev->d->download(avg != 0, f, x, z));
+}
+int tpdhg_dev_spmv(tpdhg_dev* dev, int32_t transpose, const double* v, double* y) {
+    DEV_CALL(dev, dev->d->spmv(transpose ? 1 : 0, v, y));
+}
+int tpdhg_dev_info(tpdhg_dev* dev, int64_t* mem_used, char* name_buf, int32_t name_len) {
+    DEV_CALL(dev, {
+        if (mem_used) *mem_used = int64_t(dev->d->mem_used());
+        if (name_buf && name_len > 0) std::snprintf(name_buf, size_t(name_len), "%s", dev->d->name().c_str());
+    });
+}
+int tpdhg_dev_set_profile(tpdhg_dev* dev, int32_t on) { DEV_CALL(dev, dev->d->set_profile(on != 0)); }
+int tpdhg_dev_kernel_times(tpdhg_dev* dev, double* row_ms, double* col_ms, int64_t* row_launches,
+                           int64_t* col_launches) {
+    DEV_CALL(dev, {
+        tpdhg::KernelTimes t = dev->d->kernel_times();
+        if (row_ms) *row_ms = t.row_ms;
+        if (col_ms) *col_ms = t.col_ms;
+        if (row_launches) *row_launches = t.row_launches;
+        if (col_launches) *col_launches = t.col_launches;
+    });
+}
+void tpdhg_dev_destroy(tpdhg_dev* dev) { delete dev; }
+
+void tpdhg_options_default(tpdhg_options* o) {
+    if (!o) return;
+    tpdhg::Options d;
+    o->tol = d.tol, o->time_limit = d.time_limit, o->max_iter = d.max_iter, o->eval_freq = d.eval_freq;
+    o->row_rel = d.row_rel, o->precision_bits = d.precision_bits, o->refine = d.refine ? 1 : 0;
+    o->inner_tol = d.inner_tol, o->max_refinements = d.max_refinements, o->use_graph = d.graph ? 1 : 0;
+    o->verbose = 0, o->profile_kernels = 0;
+}
+
+namespace {
+void fill_result(const tpdhg::Report& r, const tpdhg::LP& lp, tpdhg_result* res, double* x, double* y) {
+    std::memset(res, 0, sizeof(*res));
+    res->status = status_code(r.out.status);
+    res->iterations = r.out.iterations, res->restarts = r.out.restarts, res->refinements = r.out.refinements;
+    res->from_average = r.out.from_average;
+    res->objective = r.objective;
+    res->rel_primal = r.out.rel_primal, res->rel_dual = r.out.rel_dual, res->rel_gap = r.out.rel_gap;
+    res->max_row_violation = r.max_row_violation, res->max_bound_violation = r.max_bound_violation;
+    res->setup_s = r.setup_s, res->parse_s = r.tm.parse, res->scale_s = r.tm.scale;
+    res->device_init_s = r.tm.init, res->transfer_s = r.tm.transfer, res->solve_s = r.solve_s;
+    res->power_iter_s = r.out.power_s, res->total_s = r.total_s;
+    res->norm_a = r.out.norm_a, res->primal_weight = r.out.omega;
+    res->device_mem_bytes = int64_t(r.device_mem_bytes);
+    res->m = lp.m, res->n = lp.n, res->nnz = r.nnz;
+    res->row_kernel_ms = r.out.kernels.row_ms, res->col_kernel_ms = r.out.kernels.col_ms;
+    res->row_launches = r.out.kernels.row_launches, res->col_launches = r.out.kernels.col_launches;
+    std::snprintf(res->device, sizeof res->device, "%s", r.device_name.c_str());
+    if (x) std::memcpy(x, r.x.data(), r.x.size() * sizeof(double));
+    if (y) std::memcpy(y, r.y.data(), r.y.size() * sizeof(double));
+}
+
+int check_options(const tpdhg::Options& o) {
+    if (o.precision_bits != 32 && o.precision_bits != 64) return fail(TPDHG_ERR_ARG, "precision_bits must be 32 or 64");
+    if (o.refine && o.precision_bits != 32) return fail(TPDHG_ERR_ARG, "refine needs precision_bits 32");
+    return TPDHG_OK;
+}
+}  // namespace
+
+int tpdhg_solve(const tpdhg_model* model, const tpdhg_options* opt, const double* x0, const double* y0,
+                tpdhg_result* res, double* x, double* y) {
+    if (!model || !res) return fail(TPDHG_ERR_ARG, "null argument");
+    if (const char* why = tpdhg::input_error(model->lp)) return fail(TPDHG_ERR_UNSUPPORTED, why);
+    tpdhg::Options o = to_options(opt);
+    if (o.eval_freq < 1) o.eval_freq = 1;
+    if (int rc = check_options(o)) return rc;
+    return guarded([&]() -> int {
+        const tpdhg::LP& lp = model->lp;
+        tpdhg::Report r = tpdhg::solve_lp(lp, o, tpdhg::Timings{}, std::chrono::steady_clock::now(), x0, y0);
+        fill_result(r, lp, res, x, y);
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_session_create(const tpdhg_model* model, const tpdhg_options* opt, tpdhg_session** out) {
+    if (!model || !out) return fail(TPDHG_ERR_ARG, "null argument");
+    if (const char* why = tpdhg::input_error(model->lp)) return fail(TPDHG_ERR_UNSUPPORTED, why);
+    tpdhg::Options o = to_options(opt);
+    if (o.eval_freq < 1) o.eval_freq = 1;
+    if (int rc = check_options(o)) return rc;
+    return guarded([&]() -> int {
+        auto s = std::make_unique<tpdhg_session>();
+        s->lp = model->lp;
+        s->s = std::make_unique<tpdhg::Session>(s->lp, o);
+        *out = s.release();
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_session_solve(tpdhg_session* s, const double* x0, const double* y0, tpdhg_result* res, double* x, double* y) {
+    if (!s || !s->s || !res) return fail(TPDHG_ERR_ARG, "null argument");
+    return guarded([&]() -> int {
+        tpdhg::Report r = s->s->solve(x0, y0);
+        fill_result(r, s->lp, res, x, y);
+        return TPDHG_OK;
+    });
+}
+
+void tpdhg_session_free(tpdhg_session* s) { delete s; }
+
+int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const tpdhg_result* res, char* buf,
+                      int32_t len) {
+    if (!model || !res || !buf || len <= 0) return fail(TPDHG_ERR_ARG, "null argument");
+    return guarded([&]() -> int {
+        tpdhg::Report r;
+        r.out.status = tpdhg_status_string(res->status);
+        r.out.iterations = res->iterations, r.out.restarts = res->restarts, r.out.refinements = res->refinements;
+        r.out.from_average = res->from_average != 0;
+        r.out.rel_primal = res->rel_primal, r.out.rel_dual = res->rel_dual, r.out.rel_gap = res->rel_gap;
+        r.out.norm_a = res->norm_a, r.out.omega = res->primal_weight, r.out.power_s = res->power_iter_s;
+        r.objective = res->objective, r.max_row_violation = res->max_row_violation;
+        r.max_bound_violation = res->max_bound_violation;
+        r.setup_s = res->setup_s, r.solve_s = res->solve_s, r.total_s = res->total_s;
+        r.tm.parse = res->parse_s, r.tm.scale = res->scale_s, r.tm.init = res->device_init_s;
+        r.tm.transfer = res->transfer_s;
+        r.device_name = res->device, r.device_mem_bytes = size_t(res->device_mem_bytes), r.nnz = res->nnz;
+        std::string s = tpdhg::report_json(r, model->lp, to_options(opt));
+        std::snprintf(buf, size_t(len), "%s", s.c_str());
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_spmv_bench(int32_t precision_bits, int32_t rows, int32_t cols, const int32_t* ptr, const int32_t* idx,
+                     const double* val, const double* v, double* y, int32_t reps, double* mean_ms, double* median_ms) {
+    if (rows < 0 || cols < 0 || !ptr || !v || !y || !mean_ms || !median_ms || reps < 1)
+        return fail(TPDHG_ERR_ARG, "bad arguments");
+    return guarded([&]() -> int {
+        tpdhg::spmv_bench(precision_bits, rows, cols, ptr, idx, val, v, y, reps, mean_ms, median_ms);
+        return TPDHG_OK;
+    });
+}
+
+}  // extern "C"

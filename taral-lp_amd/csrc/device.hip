@@ -1,0 +1,724 @@
+// HIP kernels and device backend of the restarted PDHG path, written for gfx950 (CDNA4, wave64).
+//
+// SpMV design (shared by A x, A^T z, and the fused PDHG steps): rows are grouped on the host into row
+// blocks of at most kCap nonzeros and kWG rows. One 256-thread workgroup takes one block: it streams the
+// block's val/idx with unit-stride loads (each wave64 instruction reads 256 contiguous bytes), gathers the
+// vector entries, stages the products in LDS, then reduces each row with a power-of-two group of lanes
+// sized to the block (xor shuffles inside a wave) and applies the fused epilogue (dual step, column step
+// plus next primal step, or a plain store). A row longer than kCap gets a block of its own and is reduced
+// across the whole workgroup. Blocks are mapped XCD-aware: the hardware deals workgroups round-robin over
+// the 8 XCDs, and block b is remapped so each XCD walks one contiguous range of rows (its L2 then sees a
+// contiguous slice of the output and, for banded matrices, of the input vector).
+//
+// KKT evaluation is one fused pass over rows and columns for the current and the averaged point together,
+// FP64 accumulation from the iterate type, per-workgroup partials and a fixed-order final sum, so results
+// are deterministic run to run. The reference computes these in gpu/pdhg.cu:236-291 with one pass per point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+
+namespace tpdhg {
+namespace {
+
+constexpr int kWG = 256;    // threads per workgroup: 4 waves of 64
+constexpr int kCap = 2048;  // nonzeros staged per row block (8 per thread)
+constexpr int kMaxRedBlocks = 1024;
+
+#define HIP_OK(call)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #call ")"); \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------ kernels
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+    const int q = nblk >> 3, r = nblk & 7, x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+template <class T>
+__device__ __forceinline__ T clampv(T v, T lo, T hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+template <class T>
+struct EpiStore {  // y = M v
+    T* y;
+    __device__ void load() {}
+    __device__ void operator()(int r, T s) const { y[r] = s; }
+};
+
+// Dual (row) step, the semantics of gpu/pdhg.cu:179-190: s = (A x)_i, v = z + sigma (2 s - Ax_old),
+// z+ = v - sigma proj_[rl, ru](v / sigma); Ax = s; running sums Axs += s, zs += z+.
+template <class T>
+struct EpiRow {
+    T *Ax, *z, *Axs, *zs;
+    const T *rl, *ru, *par;
+    T sig;
+    __device__ void load() { sig = par[1]; }
+    __device__ void operator()(int i, T s) const {
+        const T v = z[i] + sig * (2 * s - Ax[i]), q = v / sig;
+        const T lo = rl[i], hi = ru[i];
+        const T zn = q < lo ? v - sig * lo : (q > hi ? v - sig * hi : T(0));
+        Ax[i] = s, z[i] = zn, Axs[i] += s, zs[i] += zn;
+    }
+};
+
+// Column step, gpu/pdhg.cu:193-206: s = (A^T z)_j; ATz = s, ATzs += s; when upd, the next primal step
+// x+ = proj_[l, u](x - tau (c + s)), xs += x+.
+template <class T>
+struct EpiCol {
+    T *ATz, *ATzs, *x, *xs;
+    const T *c, *l, *u, *par;
+    int upd;
+    T tau;
+    __device__ void load() { tau = par[0]; }
+    __device__ void operator()(int j, T s) const {
+        ATz[j] = s, ATzs[j] += s;
+        if (upd) {
+            const T xn = clampv(x[j] - tau * (c[j] + s), l[j], u[j]);
+            x[j] = xn, xs[j] += xn;
+        }
+    }
+};
+
+template <class T, class Epi>
+__global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restrict__ blk, const int* __restrict__ ptr,
+                                                   const int* __restrict__ idx, const T* __restrict__ val,
+                                                   const T* __restrict__ v, Epi epi) {
+    __shared__ T sh[kCap];
+    __shared__ T red[kWG / 64];
+    const int b = xcd_block(int(blockIdx.x), nblk);
+    const int tid = int(threadIdx.x);
+    const int r0 = blk[b], r1 = blk[b + 1];
+    const int k0 = ptr[r0], k1 = ptr[r1];
+    epi.load();
+    if (k1 - k0 > kCap) {  // a single long row: strided partial sums and a workgroup reduction
+        T s = 0;
+        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * v[idx[k]];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) epi(r0, (red[0] + red[1]) + (red[2] + red[3]));
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < kCap / kWG; ++u) {
+        const int k = k0 + u * kWG + tid;
+        if (k < k1) sh[k - k0] = val[k] * v[idx[k]];
+    }
+    __syncthreads();
+    const int nr = r1 - r0;
+    int lg = 6;  // lanes per row = 2^lg: the widest group with nr groups fitting the workgroup
+    while (lg > 0 && (nr << lg) > kWG) --lg;
+    const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
+    T s = 0;
+    if (row < nr) {
+        const int a = ptr[r0 + row] - k0, e = ptr[r0 + row + 1] - k0;
+        for (int k = a + lane; k < e; k += lpr) s += sh[k];
+    }
+    for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (row < nr && lane == 0) epi(r0 + row, s);
+}
+
+// First primal step of a block from the stored A^T z (gpu/pdhg.cu:208-214).
+template <class T>
+__global__ __launch_bounds__(kWG) void k_xupd(int n, T* __restrict__ x, const T* __restrict__ ATz,
+                                              const T* __restrict__ c, const T* __restrict__ l,
+                                              const T* __restrict__ u, const T* __restrict__ par,
+                                              T* __restrict__ xs) {
+    const T tau = par[0];
+    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x)) {
+        const T xn = clampv(x[j] - tau * (c[j] + ATz[j]), l[j], u[j]);
+        x[j] = xn, xs[j] += xn;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    return v;
+}
+
+// Workgroup sum of K doubles per thread into part[blockIdx * K + k] (lane order fixed: deterministic).
+template <int K>
+__device__ __forceinline__ void block_store(double (&v)[K], double* __restrict__ part) {
+    __shared__ double sh[K][kWG / 64];
+    const int lane = int(threadIdx.x) & 63, w = int(threadIdx.x) >> 6;
+    for (int k = 0; k < K; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) sh[k][w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        const int k = int(threadIdx.x);
+        double s = 0;
+        for (int q = 0; q < kWG / 64; ++q) s += sh[k][q];
+        part[size_t(blockIdx.x) * K + k] = s;
+    }
+}
+
+__device__ __forceinline__ double row_rel_weight(double lo, double up, double dr) {
+    double b = 0;
+    if (lo > -INFINITY) b = lo < 0 ? -lo : lo;
+    if (up < INFINITY) {
+        const double a = up < 0 ? -up : up;
+        b = a > b ? a : b;
+    }
+    return 1.0 / (1.0 + b / dr);
+}
+
+template <class T>
+struct EvalArgs {
+    const T *Ax, *z, *Axs, *zs, *rl, *ru, *Dr, *zlast;
+    const T *x, *ATz, *xs, *ATzs, *c, *l, *u, *Dc, *xlast;
+};
+
+// KKT scalars of the current point and of the f-scaled average in one pass (gpu/pdhg.cu:236-271 twice).
+// Blocks [0, nbr) cover rows, blocks [nbr, grid) cover columns; each writes 10 partials:
+// rows: current PR2S..DZ2 then average PR2S..DZ2; columns: current DR2S..DX2 then average DR2S..DX2.
+template <class T>
+__global__ __launch_bounds__(kWG) void k_eval2(int m, int n, int nbr, double f, EvalArgs<T> a,
+                                               double* __restrict__ part) {
+    double v[10];
+    for (int k = 0; k < 10; ++k) v[k] = 0;
+    const int b = int(blockIdx.x);
+    if (b < nbr) {
+        for (int i = b * kWG + int(threadIdx.x); i < m; i += nbr * kWG) {
+            const double lo = a.rl[i], up = a.ru[i], dr = a.Dr[i], zl = a.zlast[i];
+            const double wrel = row_rel_weight(lo, up, dr);
+            for (int h = 0; h < 2; ++h) {
+                const double ax = h ? a.Axs[i] * f : double(a.Ax[i]), zz = h ? a.zs[i] * f : double(a.z[i]);
+                const double r = ax < lo ? lo - ax : (ax > up ? ax - up : 0), ro = r / dr, rr = ro * wrel;
+                double* o = v + 5 * h;
+                o[0] += r * r, o[1] += ro * ro, o[2] += rr * rr;
+                if (zz > 0 && up < INFINITY) o[3] -= zz * up;
+                if (zz < 0 && lo > -INFINITY) o[3] -= zz * lo;
+                const double dz = zz - zl;
+                o[4] += dz * dz;
+            }
+        }
+    } else {
+        const int nbc = int(gridDim.x) - nbr;
+        for (int j = (b - nbr) * kWG + int(threadIdx.x); j < n; j += nbc * kWG) {
+            const double cj = a.c[j], lo = a.l[j], up = a.u[j], dc = a.Dc[j], xl = a.xlast[j];
+            for (int h = 0; h < 2; ++h) {
+                const double xx = h ? a.xs[j] * f : double(a.x[j]);
+                const double d = cj + (h ? a.ATzs[j] * f : double(a.ATz[j]));
+                double res = 0;
+                double* o = v + 5 * h;
+                o[2] += cj * xx;
+                if (d > 0) {
+                    if (lo > -INFINITY) o[3] += d * lo;
+                    else res = d;
+                } else if (d < 0) {
+                    if (up < INFINITY) o[3] += d * up;
+                    else res = -d;
+                }
+                const double ro = res / dc, dx = xx - xl;
+                o[0] += res * res, o[1] += ro * ro, o[4] += dx * dx;
+            }
+        }
+    }
+    block_store<10>(v, part);
+}
+
+// Fixed-order final sums: out[0..9] = current KKT, out[10..19] = average KKT.
+__global__ void k_eval2_finish(int nbr, int nbc, const double* __restrict__ part, double* __restrict__ out) {
+    const int k = int(threadIdx.x);
+    if (k >= 10) return;
+    double sr = 0, sc = 0;
+    for (int q = 0; q < nbr; ++q) sr += part[size_t(q) * 10 + k];
+    for (int q = nbr; q < nbr + nbc; ++q) sc += part[size_t(q) * 10 + k];
+    const int h = k / 5, s = k % 5;
+    out[10 * h + s] = sr, out[10 * h + 5 + s] = sc;
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_sumsq(int n, const T* __restrict__ x, double* __restrict__ part) {
+    double v[1] = {0};
+    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
+        v[0] += double(x[j]) * double(x[j]);
+    block_store<1>(v, part);
+}
+
+__global__ void k_sum_finish(int blocks, const double* __restrict__ part, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double s = 0;
+    for (int q = 0; q < blocks; ++q) s += part[q];
+    out[0] = s;
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_scale(int n, T* __restrict__ x, double f) {
+    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
+        x[j] = T(x[j] * f);
+}
+
+// Restart, gpu/pdhg.cu:293-299: x <- average (xs f) or current; xlast = x; clear both running sums.
+template <class T>
+__global__ __launch_bounds__(kWG) void k_restart(int n, int avg, double f, T* __restrict__ x, T* __restrict__ xs,
+                                                 T* __restrict__ xlast, T* __restrict__ ys) {
+    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x)) {
+        const T v = avg ? T(xs[j] * f) : x[j];
+        x[j] = v, xlast[j] = v, xs[j] = 0, ys[j] = 0;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_to_f64(int n, double f, const T* __restrict__ src, double* __restrict__ dst) {
+    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
+        dst[j] = double(src[j]) * f;
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_from_f64(int n, const double* __restrict__ src, T* __restrict__ dst) {
+    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
+        dst[j] = T(src[j]);
+}
+
+// ------------------------------------------------------------------------------------------------ host side
+// Row blocks: maximal runs of rows holding at most kCap nonzeros and at most kWG rows; a row longer than
+// kCap forms a block of its own.
+std::vector<int32_t> row_blocks(int rows, const int32_t* ptr) {
+    std::vector<int32_t> b;
+    b.push_back(0);
+    int r0 = 0;
+    while (r0 < rows) {
+        int r = r0;
+        while (r < rows && r - r0 < kWG && int64_t(ptr[r + 1]) - ptr[r0] <= kCap) ++r;
+        if (r == r0) r = r0 + 1;  // long row
+        b.push_back(r);
+        r0 = r;
+    }
+    return b;
+}
+
+int grid_for(int k) { return std::max(1, std::min(2048, (k + kWG - 1) / kWG)); }
+int red_blocks(int k) { return k > 0 ? std::max(1, std::min(kMaxRedBlocks, (k + kWG - 1) / kWG)) : 0; }
+
+struct Mem {  // device allocations owned by one backend
+    std::vector<void*> ptrs;
+    template <class U>
+    U* alloc(size_t count) {
+        void* p = nullptr;
+        HIP_OK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(U)));
+        ptrs.push_back(p);
+        return static_cast<U*>(p);
+    }
+    ~Mem() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+template <class T>
+struct DevMat {
+    int rows = 0, nblk = 0;
+    int *ptr = nullptr, *idx = nullptr, *blk = nullptr;
+    T* val = nullptr;
+};
+
+template <class T>
+DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, const int32_t* ptr, const int32_t* idx, const double* val) {
+    DevMat<T> d;
+    d.rows = rows;
+    const int64_t nnz = ptr[rows];
+    std::vector<int32_t> blk = row_blocks(rows, ptr);
+    d.nblk = int(blk.size()) - 1;
+    d.ptr = mem.alloc<int>(size_t(rows) + 1), d.idx = mem.alloc<int>(size_t(nnz)), d.blk = mem.alloc<int>(blk.size());
+    d.val = mem.alloc<T>(size_t(nnz));
+    HIP_OK(hipMemcpyAsync(d.ptr, ptr, (size_t(rows) + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+    if (nnz) HIP_OK(hipMemcpyAsync(d.idx, idx, size_t(nnz) * sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d.blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (nnz) {
+        std::vector<T> h(val, val + nnz);
+        HIP_OK(hipMemcpyAsync(d.val, h.data(), size_t(nnz) * sizeof(T), hipMemcpyHostToDevice, st));
+        HIP_OK(hipStreamSynchronize(st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    return d;
+}
+
+template <class T, class Epi>
+void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
+    if (a.nblk == 0) return;
+    k_rowblocks<T, Epi><<<a.nblk, kWG, 0, st>>>(a.nblk, a.blk, a.ptr, a.idx, a.val, v, epi);
+}
+
+std::string device_name() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, dev));
+    std::string nm = prop.name;
+    if (nm.empty()) nm = prop.gcnArchName;
+    return nm;
+}
+
+void require_device() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        throw std::runtime_error("no HIP device visible: the PDHG solver runs on an AMD GPU only (there is no CPU path)");
+}
+
+template <class T>
+class Dev final : public Device {
+public:
+    Dev(int m, int n, const int32_t* a_ptr, const int32_t* a_idx, const double* a_val, const int32_t* t_ptr,
+        const int32_t* t_idx, const double* t_val, const double* c_, const double* l_, const double* u_,
+        const double* rl_, const double* ru_, const double* Dr_, const double* Dc_, const DeviceOptions& opt)
+        : m_(m), n_(n), eval_freq_(opt.eval_freq), use_graph_(opt.use_graph) {
+        require_device();
+        size_t free0 = 0, free1 = 0, total = 0;
+        HIP_OK(hipMemGetInfo(&free0, &total));
+        name_ = device_name();
+        HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        A_ = upload_csr<T>(mem_, st_, m, a_ptr, a_idx, a_val);
+        AT_ = upload_csr<T>(mem_, st_, n, t_ptr, t_idx, t_val);
+        hl_.assign(l_, l_ + n), hu_.assign(u_, u_ + n);
+        c = up(c_, n), l = up(l_, n), u = up(u_, n), rl = up(rl_, m), ru = up(ru_, m), Dr = up(Dr_, m), Dc = up(Dc_, n);
+        par = zeros(2);
+        x = zeros(n), ATz = zeros(n), xs = zeros(n), ATzs = zeros(n), xlast = zeros(n);
+        z = zeros(m), Ax = zeros(m), zs = zeros(m), Axs = zeros(m), zlast = zeros(m);
+        pv = zeros(n), pw = zeros(m), pu = zeros(n);
+        nbr_ = red_blocks(m), nbc_ = red_blocks(n);
+        part_ = mem_.alloc<double>(size_t(std::max(1, nbr_ + nbc_)) * 10 + size_t(kMaxRedBlocks));
+        out_ = mem_.alloc<double>(32);
+        f64buf_ = mem_.alloc<double>(size_t(std::max(m, n)));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hout_), 32 * sizeof(double), hipHostMallocDefault));
+        HIP_OK(hipStreamSynchronize(st_));
+        HIP_OK(hipMemGetInfo(&free1, &total));
+        mem_used_ = free0 > free1 ? free0 - free1 : 0;
+    }
+    ~Dev() override {
+        if (graph_) (void)hipGraphExecDestroy(graph_);
+        for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+        if (hout_) (void)hipHostFree(hout_);
+        if (st_) (void)hipStreamDestroy(st_);
+    }
+
+    double power_norm(int iters) override {
+        std::vector<double> h(static_cast<size_t>(n_));
+        double nn = 0;
+        for (int j = 0; j < n_; ++j) {
+            h[size_t(j)] = 0.5 + double((uint32_t(j) * 2654435761u) % 1000u) / 1000.0;  // gpu/pdhg.cu:153
+            nn += h[size_t(j)] * h[size_t(j)];
+        }
+        std::vector<T> ht(static_cast<size_t>(n_));
+        for (int j = 0; j < n_; ++j) ht[size_t(j)] = T(h[size_t(j)] / std::sqrt(nn));
+        h2d(pv, ht.data(), size_t(n_));
+        double est = 0;
+        for (int it = 0; it < iters; ++it) {
+            spmv_dev(A_, pv, pw);
+            spmv_dev(AT_, pw, pu);
+            const double s = sumsq(n_, pu), prev = est;
+            if (s <= 0) return 0;
+            est = std::sqrt(std::sqrt(s));
+            k_scale<T><<<grid_for(n_), kWG, 0, st_>>>(n_, pu, 1 / std::sqrt(s));
+            std::swap(pv, pu);
+            if (it >= 10 && std::fabs(est - prev) <= 1e-7 * est) break;
+        }
+        return est;
+    }
+
+    void set_steps(double tau, double sigma) override {
+        T h[2] = {T(tau), T(sigma)};
+        h2d(par, h, 2);
+    }
+
+    void start(const double* x_init, const double* z_init) override {
+        // x0 = clamp(x_init or 0, l, u) in FP64 on the host with the FP64 bounds (gpu/pdhg.cu:452), z0 = z_init or 0;
+        // the running sums are cleared so a handle can be started again.
+        std::vector<T> hx(static_cast<size_t>(n_)), hz(static_cast<size_t>(m_), T(0));
+        for (int j = 0; j < n_; ++j) {
+            const double v = x_init ? x_init[j] : 0.0;
+            hx[size_t(j)] = T(std::min(std::max(v, hl_[size_t(j)]), hu_[size_t(j)]));
+        }
+        if (z_init)
+            for (int i = 0; i < m_; ++i) hz[size_t(i)] = T(z_init[i]);
+        h2d(x, hx.data(), size_t(n_)), h2d(xlast, hx.data(), size_t(n_));
+        h2d(z, hz.data(), size_t(m_)), h2d(zlast, hz.data(), size_t(m_));
+        for (T* p : {xs, ATzs}) HIP_OK(hipMemsetAsync(p, 0, size_t(std::max(n_, 1)) * sizeof(T), st_));
+        for (T* p : {zs, Axs}) HIP_OK(hipMemsetAsync(p, 0, size_t(std::max(m_, 1)) * sizeof(T), st_));
+        products();
+        HIP_OK(hipStreamSynchronize(st_));
+        HIP_OK(hipGetLastError());
+    }
+
+    void iterate(int steps) override {
+        if (steps <= 0) return;
+        if (use_graph_ && !profile_ && steps == eval_freq_) {
+            if (!graph_) {
+                hipGraph_t g = nullptr;
+                HIP_OK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+                try {
+                    launch_block(steps);
+                } catch (...) {
+                    (void)hipStreamEndCapture(st_, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                HIP_OK(hipStreamEndCapture(st_, &g));
+                HIP_OK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+                HIP_OK(hipGraphDestroy(g));
+            }
+            HIP_OK(hipGraphLaunch(graph_, st_));
+        } else {
+            launch_block(steps);
+        }
+        HIP_OK(hipGetLastError());
+    }
+
+    void eval2(double f, double* cur, double* avg) override {
+        if (nbr_ + nbc_ > 0) {
+            EvalArgs<T> a{Ax, z, Axs, zs, rl, ru, Dr, zlast, x, ATz, xs, ATzs, c, l, u, Dc, xlast};
+            k_eval2<T><<<nbr_ + nbc_, kWG, 0, st_>>>(m_, n_, nbr_, f, a, part_);
+            k_eval2_finish<<<1, 64, 0, st_>>>(nbr_, nbc_, part_, out_);
+            HIP_OK(hipMemcpyAsync(hout_, out_, 20 * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+        } else {
+            std::fill(hout_, hout_ + 20, 0.0);
+        }
+        HIP_OK(hipGetLastError());
+        if (cur) std::copy(hout_, hout_ + 10, cur);
+        if (avg) std::copy(hout_ + 10, hout_ + 20, avg);
+        collect_events();
+    }
+
+    void restart(bool avg, double f) override {
+        k_restart<T><<<grid_for(n_), kWG, 0, st_>>>(n_, avg ? 1 : 0, f, x, xs, xlast, ATzs);
+        k_restart<T><<<grid_for(m_), kWG, 0, st_>>>(m_, avg ? 1 : 0, f, z, zs, zlast, Axs);
+        products();
+        HIP_OK(hipGetLastError());
+    }
+
+    void download(bool avg, double f, double* hx, double* hz) override {
+        if (hx && n_) {
+            k_to_f64<T><<<grid_for(n_), kWG, 0, st_>>>(n_, avg ? f : 1.0, avg ? xs : x, f64buf_);
+            HIP_OK(hipMemcpyAsync(hx, f64buf_, size_t(n_) * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+        }
+        if (hz && m_) {
+            k_to_f64<T><<<grid_for(m_), kWG, 0, st_>>>(m_, avg ? f : 1.0, avg ? zs : z, f64buf_);
+            HIP_OK(hipMemcpyAsync(hz, f64buf_, size_t(m_) * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+        }
+    }
+
+    void spmv(int which, const double* v, double* y) override {
+        const DevMat<T>& a = which ? AT_ : A_;
+        const int cols = which ? m_ : n_, rows = which ? n_ : m_;
+        T* in = which ? pw : pv;   // pv holds n entries, pw m, pu n
+        T* res = which ? pu : pw;
+        std::vector<T> h(v, v + cols);
+        h2d(in, h.data(), size_t(cols));
+        spmv_dev(a, in, res);
+        if (rows) {
+            k_to_f64<T><<<grid_for(rows), kWG, 0, st_>>>(rows, 1.0, res, f64buf_);
+            HIP_OK(hipMemcpyAsync(y, f64buf_, size_t(rows) * sizeof(double), hipMemcpyDeviceToHost, st_));
+        }
+        HIP_OK(hipStreamSynchronize(st_));
+        HIP_OK(hipGetLastError());
+    }
+
+    void set_profile(bool on) override { profile_ = on; }
+    KernelTimes kernel_times() override {
+        HIP_OK(hipStreamSynchronize(st_));
+        collect_events();
+        KernelTimes t = times_;
+        times_ = KernelTimes();
+        return t;
+    }
+    void synchronize() override { HIP_OK(hipStreamSynchronize(st_)); }
+    const std::string& name() const override { return name_; }
+    size_t mem_used() const override { return mem_used_; }
+    int precision_bits() const override { return int(sizeof(T) * 8); }
+    int m() const override { return m_; }
+    int n() const override { return n_; }
+
+private:
+    T* up(const double* src, int k) {
+        T* p = mem_.alloc<T>(size_t(k));
+        if (k) {
+            std::vector<T> h(src, src + k);
+            HIP_OK(hipMemcpyAsync(p, h.data(), size_t(k) * sizeof(T), hipMemcpyHostToDevice, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+        }
+        return p;
+    }
+    T* zeros(int k) {
+        T* p = mem_.alloc<T>(size_t(k));
+        HIP_OK(hipMemsetAsync(p, 0, std::max<size_t>(size_t(k), 1) * sizeof(T), st_));
+        return p;
+    }
+    void h2d(T* dst, const T* src, size_t k) {
+        if (!k) return;
+        HIP_OK(hipMemcpyAsync(dst, src, k * sizeof(T), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+    void spmv_dev(const DevMat<T>& a, const T* v, T* y) { launch_rows(a, v, EpiStore<T>{y}, st_); }
+    double sumsq(int k, const T* v) {
+        const int b = red_blocks(k);
+        if (!b) return 0;
+        k_sumsq<T><<<b, kWG, 0, st_>>>(k, v, part_);
+        k_sum_finish<<<1, 64, 0, st_>>>(b, part_, out_);
+        HIP_OK(hipMemcpyAsync(hout_, out_, sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        return hout_[0];
+    }
+    void products() {  // Ax = A x, ATz = A^T z exactly
+        spmv_dev(A_, x, Ax);
+        spmv_dev(AT_, z, ATz);
+    }
+    hipEvent_t event(size_t i) {
+        while (events_.size() <= i) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            events_.push_back(e);
+        }
+        return events_[i];
+    }
+    void launch_block(int steps) {
+        k_xupd<T><<<grid_for(n_), kWG, 0, st_>>>(n_, x, ATz, c, l, u, par, xs);
+        for (int s = 0; s < steps; ++s) {
+            EpiRow<T> er{Ax, z, Axs, zs, rl, ru, par, T(0)};
+            EpiCol<T> ec{ATz, ATzs, x, xs, c, l, u, par, s + 1 < steps ? 1 : 0, T(0)};
+            if (profile_) {
+                const size_t e = pending_.size();
+                HIP_OK(hipEventRecord(event(e), st_));
+                launch_rows(A_, x, er, st_);
+                HIP_OK(hipEventRecord(event(e + 1), st_));
+                launch_rows(AT_, z, ec, st_);
+                HIP_OK(hipEventRecord(event(e + 2), st_));
+                pending_.push_back(0), pending_.push_back(1), pending_.push_back(2);
+            } else {
+                launch_rows(A_, x, er, st_);
+                launch_rows(AT_, z, ec, st_);
+            }
+        }
+    }
+    void collect_events() {
+        if (pending_.empty()) return;
+        HIP_OK(hipStreamSynchronize(st_));
+        for (size_t i = 0; i + 2 < pending_.size(); i += 3) {
+            float a = 0, b = 0;
+            HIP_OK(hipEventElapsedTime(&a, events_[i], events_[i + 1]));
+            HIP_OK(hipEventElapsedTime(&b, events_[i + 1], events_[i + 2]));
+            times_.row_ms += a, times_.col_ms += b;
+            times_.row_launches++, times_.col_launches++;
+        }
+        pending_.clear();
+    }
+
+    int m_, n_, eval_freq_;
+    bool use_graph_, profile_ = false;
+    Mem mem_;
+    hipStream_t st_ = nullptr;
+    hipGraphExec_t graph_ = nullptr;
+    DevMat<T> A_, AT_;
+    T *c, *l, *u, *rl, *ru, *Dr, *Dc, *par;
+    T *x, *z, *Ax, *ATz, *xs, *zs, *Axs, *ATzs, *xlast, *zlast;
+    T *pv, *pw, *pu;
+    double *part_ = nullptr, *out_ = nullptr, *f64buf_ = nullptr, *hout_ = nullptr;
+    int nbr_ = 0, nbc_ = 0;
+    size_t mem_used_ = 0;
+    std::string name_;
+    std::vector<double> hl_, hu_;
+    std::vector<hipEvent_t> events_;
+    std::vector<int> pending_;
+    KernelTimes times_;
+};
+
+}  // namespace
+
+int device_count() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+    return count;
+}
+
+std::unique_ptr<Device> make_device_raw(int m, int n, int64_t nnz, const int32_t* a_ptr, const int32_t* a_idx,
+                                        const double* a_val, const int32_t* t_ptr, const int32_t* t_idx,
+                                        const double* t_val, const double* c, const double* l, const double* u,
+                                        const double* rl, const double* ru, const double* Dr, const double* Dc,
+                                        const DeviceOptions& opt) {
+    if (m < 0 || n < 0 || nnz < 0 || nnz > INT32_MAX) throw std::runtime_error("problem dimensions out of range");
+    if (a_ptr[m] != nnz || t_ptr[n] != nnz) throw std::runtime_error("CSR row pointers do not match nnz");
+    if (opt.eval_freq < 1) throw std::runtime_error("eval_freq must be >= 1");
+    if (opt.precision_bits == 32)
+        return std::unique_ptr<Device>(
+            new Dev<float>(m, n, a_ptr, a_idx, a_val, t_ptr, t_idx, t_val, c, l, u, rl, ru, Dr, Dc, opt));
+    if (opt.precision_bits == 64)
+        return std::unique_ptr<Device>(
+            new Dev<double>(m, n, a_ptr, a_idx, a_val, t_ptr, t_idx, t_val, c, l, u, rl, ru, Dr, Dc, opt));
+    throw std::runtime_error("precision must be 32 or 64 bits");
+}
+
+std::unique_ptr<Device> make_device(const Scaled& P, const DeviceOptions& opt) {
+    return make_device_raw(P.m, P.n, P.A.nnz(), P.A.ptr.data(), P.A.idx.data(), P.A.val.data(), P.AT.ptr.data(),
+                           P.AT.idx.data(), P.AT.val.data(), P.c.data(), P.l.data(), P.u.data(), P.rl.data(),
+                           P.ru.data(), P.Dr.data(), P.Dc.data(), opt);
+}
+
+namespace {
+template <class T>
+void spmv_bench_t(int rows, int cols, const int32_t* ptr, const int32_t* idx, const double* val, const double* v,
+                  double* y, int reps, double* mean_ms, double* median_ms) {
+    require_device();
+    Mem mem;
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    DevMat<T> a = upload_csr<T>(mem, st, rows, ptr, idx, val);
+    T* dv = mem.alloc<T>(size_t(cols));
+    T* dy = mem.alloc<T>(size_t(rows));
+    double* dy64 = mem.alloc<double>(size_t(rows));
+    std::vector<T> hv(v, v + cols);
+    if (cols) HIP_OK(hipMemcpyAsync(dv, hv.data(), size_t(cols) * sizeof(T), hipMemcpyHostToDevice, st));
+    launch_rows(a, dv, EpiStore<T>{dy}, st);  // warm-up
+    HIP_OK(hipStreamSynchronize(st));
+    std::vector<hipEvent_t> ev(size_t(2 * std::max(reps, 1)));
+    for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+    for (int r = 0; r < reps; ++r) {
+        HIP_OK(hipEventRecord(ev[size_t(2 * r)], st));
+        launch_rows(a, dv, EpiStore<T>{dy}, st);
+        HIP_OK(hipEventRecord(ev[size_t(2 * r + 1)], st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(hipGetLastError());
+    std::vector<double> ts;
+    for (int r = 0; r < reps; ++r) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, ev[size_t(2 * r)], ev[size_t(2 * r + 1)]));
+        ts.push_back(ms);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    double sum = 0;
+    for (double t : ts) sum += t;
+    std::sort(ts.begin(), ts.end());
+    *mean_ms = reps ? sum / reps : 0;
+    *median_ms = reps ? ts[ts.size() / 2] : 0;
+    k_to_f64<T><<<grid_for(rows), kWG, 0, st>>>(rows, 1.0, dy, dy64);
+    if (rows) HIP_OK(hipMemcpyAsync(y, dy64, size_t(rows) * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+}
+}  // namespace
+
+void spmv_bench(int precision_bits, int rows, int cols, const int32_t* ptr, const int32_t* idx, const double* val,
+                const double* v, double* y, int reps, double* mean_ms, double* median_ms) {
+    if (precision_bits == 32) spmv_bench_t<float>(rows, cols, ptr, idx, val, v, y, reps, mean_ms, median_ms);
+    else if (precision_bits == 64) spmv_bench_t<double>(rows, cols, ptr, idx, val, v, y, reps, mean_ms, median_ms);
+    else throw std::runtime_error("precision must be 32 or 64 bits");
+}
+
+}  // namespace tpdhg

@@ -1,0 +1,78 @@
+// Device backend of the PDHG path on one MI355X (gfx950): owns the scaled problem and the iterates in HBM and
+// runs the hand-written HIP kernels. It offers the operations of the reference backend concept
+// (gpu/pdhg.cu:313-514: init, power_norm, set_steps, start, iterate, eval, restart, download, name,
+// mem_used), plus a fused current+average evaluation and per-kernel event timing. No HIP types appear here.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "prep.hpp"
+
+namespace tpdhg {
+
+// KKT scalar slots of one evaluation: row part out[0..4] (gpu/pdhg.cu:136), column part out[5..9] (:145).
+enum KktSlot { PR2S = 0, PR2O, PR2R, DROW, DZ2, DR2S = 5, DR2O, POBJ, DCOL, DX2, NKKT = 10 };
+
+struct KernelTimes {  // HIP-event totals of the two iteration kernels since the last reset
+    double row_ms = 0, col_ms = 0;
+    int64_t row_launches = 0, col_launches = 0;
+};
+
+class Device {
+public:
+    virtual ~Device() = default;
+    // Power iteration estimate of ||A_s||_2 (gpu/pdhg.cu:416-436).
+    virtual double power_norm(int iters) = 0;
+    // Primal and dual step sizes, kept in device memory so a captured iteration block never changes.
+    virtual void set_steps(double tau, double sigma) = 0;
+    // x0 = clamp(x_init or 0, l, u), z0 = z_init or 0 (scaled space), then Ax, A^T z (gpu/pdhg.cu:450-458).
+    virtual void start(const double* x_init, const double* z_init) = 0;
+    // One block: first x-update, then `steps` x (dual row step, column step fused with the next x-update)
+    // (gpu/pdhg.cu:459-486).
+    virtual void iterate(int steps) = 0;
+    // KKT scalars of current (cur) and average (avg, scaled by f) in one pass; either pointer may be null.
+    virtual void eval2(double f, double* cur, double* avg) = 0;
+    // Restart to the average (avg, scaled by f) or the current point (gpu/pdhg.cu:500-504).
+    virtual void restart(bool avg, double f) = 0;
+    // Current or f-scaled average iterate in FP64, scaled space (gpu/pdhg.cu:505-513).
+    virtual void download(bool avg, double f, double* x, double* z) = 0;
+    // y = A_s v (which = 0) or A_s^T v (which = 1) on host vectors, for checks.
+    virtual void spmv(int which, const double* v, double* y) = 0;
+    virtual void set_profile(bool on) = 0;
+    virtual KernelTimes kernel_times() = 0;
+    virtual void synchronize() = 0;
+    virtual const std::string& name() const = 0;
+    virtual size_t mem_used() const = 0;
+    virtual int precision_bits() const = 0;
+    virtual int m() const = 0;
+    virtual int n() const = 0;
+};
+
+struct DeviceOptions {
+    int precision_bits = 64;  // 64: FP64 iterates; 32: FP32 iterates and SpMV, FP64 reductions
+    int eval_freq = 64;
+    bool use_graph = true;
+};
+
+// Uploads the scaled problem (A, A^T, c, l, u, rl, ru, Dr, Dc) to the current HIP device. Throws
+// std::runtime_error on any HIP failure; with no HIP device the error says so (there is no CPU path).
+std::unique_ptr<Device> make_device(const Scaled& P, const DeviceOptions& opt);
+
+// Same from raw scaled arrays (A and A^T as CSR, int32 indices).
+std::unique_ptr<Device> make_device_raw(int m, int n, int64_t nnz, const int32_t* a_ptr, const int32_t* a_idx,
+                                        const double* a_val, const int32_t* t_ptr, const int32_t* t_idx,
+                                        const double* t_val, const double* c, const double* l, const double* u,
+                                        const double* rl, const double* ru, const double* Dr, const double* Dc,
+                                        const DeviceOptions& opt);
+
+// Number of visible HIP devices (0 when none); never throws.
+int device_count();
+
+// Times y = M v for a CSR on the device with the solver's SpMV kernel: `reps` launches after one warm-up,
+// returns the mean and the median kernel time in ms (HIP events) and y (FP64).
+void spmv_bench(int precision_bits, int rows, int cols, const int32_t* ptr, const int32_t* idx, const double* val,
+                const double* v, double* y, int reps, double* mean_ms, double* median_ms);
+
+}  // namespace tpdhg

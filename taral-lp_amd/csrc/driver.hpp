@@ -1,0 +1,104 @@
+// Restarted averaged PDHG driver and solve pipeline of the PDHG path: flat LP model, options, the host
+// algorithm over the device backend (gpu/pdhg.cu:743-849), and the original-space report
+// (gpu/pdhg.cu:858-911).
+#pragma once
+#include <chrono>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "model.hpp"
+#include "prep.hpp"
+
+namespace tpdhg {
+
+// Flat column-wise LP (CSC, rows ascending per column) with the Model fields the PDHG path reads.
+struct LP {
+    std::string name;
+    int m = 0, n = 0;
+    std::vector<int64_t> colptr{0};
+    std::vector<int32_t> rowidx;
+    std::vector<double> val, cost, col_lo, col_up, row_lo, row_up;
+    std::vector<std::string> row_names, col_names;  // empty for array-built models
+    std::vector<char> is_int;
+    size_t qobj_terms = 0, sentinel_bounds = 0;
+    double obj_const = 0;
+    bool maximize = false;
+    int64_t nnz() const { return colptr.back(); }
+};
+
+LP flatten(const Model& md);
+// Reason string when PDHG must refuse the model (quadratic or integer), nullptr otherwise.
+const char* input_error(const LP& lp);
+
+struct Options {  // defaults of gpu/pdhg.cu:744-750 plus this build's additions
+    double tol = 1e-4, time_limit = 3600;
+    int64_t max_iter = 2000000000LL;
+    int eval_freq = 64;
+    double row_rel = 0;
+    bool verbose = false, graph = true;
+    int precision_bits = 64;   // 32: FP32 iterates and SpMV, FP64 reductions (the reference's --fp32)
+    bool refine = false;       // FP64 iterative refinement around FP32 inner solves
+    double inner_tol = 1e-4;   // relative KKT target of each refinement round's inner solve
+    int max_refinements = 8;
+    bool profile_kernels = false;
+};
+
+struct Outcome {
+    std::string status = "iteration_limit";
+    int64_t iterations = 0;
+    int restarts = 0, refinements = 0;
+    bool from_average = false;
+    double rel_primal = 0, rel_dual = 0, rel_gap = 0, norm_a = 0, omega = 1, power_s = 0, iterate_s = 0;
+    std::vector<double> x, z;  // scaled-space point
+    KernelTimes kernels;
+};
+
+// Restarted averaged PDHG on the device backend (gpu/pdhg.cu:761-849). x0/z0 (scaled space, nullable) warm start
+// the iterates; omega0 > 0 replaces the initial primal weight. norm_a > 0 skips the power iteration.
+Outcome run_pdhg(Device& dev, const Scaled& P, const Options& o, double time_budget, const double* x0 = nullptr,
+                 const double* z0 = nullptr, double omega0 = 0, double norm_a = 0);
+
+struct Timings {
+    double parse = 0, scale = 0, init = 0, transfer = 0;
+};
+
+struct Report {  // what the CLI prints and the C ABI returns (gpu/pdhg.cu:866-897)
+    Outcome out;
+    double objective = 0, max_row_violation = 0, max_bound_violation = 0;
+    double setup_s = 0, solve_s = 0, total_s = 0;
+    Timings tm;
+    std::string device_name;
+    size_t device_mem_bytes = 0;
+    int64_t nnz = 0;  // nonzeros of the scaled matrix (explicit zeros dropped)
+    std::vector<double> x, y;  // original space: x = Dc x_s, y = Dr z_s
+};
+
+// A problem resident on the device: scaled once, uploaded once, then solved any number of times from cold or
+// warm starts (each solve runs the power iteration and the PDHG loop afresh).
+class Session {
+public:
+    Session(const LP& lp, const Options& o, Timings tm = Timings());
+    Report solve(const double* x0_orig = nullptr, const double* y0_orig = nullptr);
+    const Scaled& scaled() const { return P_; }
+    Device& device() { return *dev_; }
+    const Options& options() const { return o_; }
+
+private:
+    const LP& lp_;
+    Options o_;
+    Timings tm_;
+    Scaled P_;
+    std::unique_ptr<Device> dev_;
+    double setup_s_ = 0;
+    std::chrono::steady_clock::time_point created_;
+};
+
+// Scales, uploads and solves on the current HIP device; throws std::runtime_error on device failure.
+Report solve_lp(const LP& lp, const Options& o, Timings tm, std::chrono::steady_clock::time_point t_start,
+                const double* x0_orig = nullptr, const double* y0_orig = nullptr);
+
+std::string report_json(const Report& r, const LP& lp, const Options& o);
+
+}  // namespace tpdhg

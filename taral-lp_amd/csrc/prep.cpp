@@ -1,0 +1,131 @@
+// Host problem preparation (gpu/pdhg.cu:48-133), written for large inputs: flat arrays, two-pass
+// counting transposes, no per-column containers after the model is flattened.
+#include "prep.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace tpdhg {
+
+Csr transpose(const Csr& a, int cols) {
+    Csr t;
+    t.rows = cols;
+    t.ptr.assign(size_t(cols) + 1, 0);
+    for (int32_t k : a.idx) t.ptr[size_t(k) + 1]++;
+    for (int j = 0; j < cols; ++j) t.ptr[size_t(j) + 1] += t.ptr[size_t(j)];
+    t.idx.resize(a.idx.size());
+    t.val.resize(a.val.size());
+    std::vector<int32_t> next(t.ptr.begin(), t.ptr.end() - 1);
+    for (int i = 0; i < a.rows; ++i)
+        for (int32_t k = a.ptr[size_t(i)]; k < a.ptr[size_t(i) + 1]; ++k) {
+            int32_t p = next[size_t(a.idx[size_t(k)])]++;
+            t.idx[size_t(p)] = i;
+            t.val[size_t(p)] = a.val[size_t(k)];
+        }
+    return t;
+}
+
+double row_bound_measure(double lo, double up) {
+    const bool fl = std::isfinite(lo), fu = std::isfinite(up);
+    if (fl && fu) return std::max(std::fabs(lo), std::fabs(up));
+    if (fl) return std::fabs(lo);
+    if (fu) return std::fabs(up);
+    return 0;
+}
+
+namespace {
+
+// Divides A by sqrt of the row and column measures and accumulates the factors (gpu/pdhg.cu:95-102).
+void divide_by_measures(Scaled& P, std::vector<double>& rs, std::vector<double>& cs) {
+    for (double& v : rs) v = v > 0 ? std::sqrt(v) : 1;
+    for (double& v : cs) v = v > 0 ? std::sqrt(v) : 1;
+    Csr& A = P.A;
+    for (int i = 0; i < P.m; ++i) {
+        const double r = rs[size_t(i)];
+        for (int32_t k = A.ptr[size_t(i)]; k < A.ptr[size_t(i) + 1]; ++k)
+            A.val[size_t(k)] /= r * cs[size_t(A.idx[size_t(k)])];
+    }
+    for (int i = 0; i < P.m; ++i) P.Dr[size_t(i)] /= rs[size_t(i)];
+    for (int j = 0; j < P.n; ++j) P.Dc[size_t(j)] /= cs[size_t(j)];
+}
+
+void finish_scaling(Scaled& P, const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                    const double* row_up, double sign) {
+    const int m = P.m, n = P.n;
+    P.Dr.assign(size_t(m), 1.0), P.Dc.assign(size_t(n), 1.0);
+    std::vector<double> rs(static_cast<size_t>(m)), cs(static_cast<size_t>(n));
+    Csr& A = P.A;
+    for (int pass = 0; pass < 10; ++pass) {  // Ruiz, infinity norm
+        std::fill(rs.begin(), rs.end(), 0.0), std::fill(cs.begin(), cs.end(), 0.0);
+        for (int i = 0; i < m; ++i)
+            for (int32_t k = A.ptr[size_t(i)]; k < A.ptr[size_t(i) + 1]; ++k) {
+                const double a = std::fabs(A.val[size_t(k)]);
+                double& r = rs[size_t(i)];
+                double& c = cs[size_t(A.idx[size_t(k)])];
+                r = std::max(r, a), c = std::max(c, a);
+            }
+        divide_by_measures(P, rs, cs);
+    }
+    std::fill(rs.begin(), rs.end(), 0.0), std::fill(cs.begin(), cs.end(), 0.0);
+    for (int i = 0; i < m; ++i)  // Pock-Chambolle, alpha = 1: l1 norms, summed in row order
+        for (int32_t k = A.ptr[size_t(i)]; k < A.ptr[size_t(i) + 1]; ++k) {
+            const double a = std::fabs(A.val[size_t(k)]);
+            rs[size_t(i)] += a, cs[size_t(A.idx[size_t(k)])] += a;
+        }
+    divide_by_measures(P, rs, cs);
+    P.AT = transpose(A, n);
+    P.c.resize(size_t(n)), P.l.resize(size_t(n)), P.u.resize(size_t(n));
+    P.rl.resize(size_t(m)), P.ru.resize(size_t(m));
+    double cn = 0, bn = 0;
+    for (int j = 0; j < n; ++j) {
+        const double c = sign * cost[j], d = P.Dc[size_t(j)];
+        cn += c * c;
+        P.c[size_t(j)] = c * d, P.l[size_t(j)] = col_lo[j] / d, P.u[size_t(j)] = col_up[j] / d;
+    }
+    for (int i = 0; i < m; ++i) {
+        const double b = row_bound_measure(row_lo[i], row_up[i]);
+        bn += b * b;
+        P.rl[size_t(i)] = row_lo[i] * P.Dr[size_t(i)], P.ru[size_t(i)] = row_up[i] * P.Dr[size_t(i)];
+    }
+    P.bnorm = std::sqrt(bn), P.cnorm = std::sqrt(cn);
+}
+
+}  // namespace
+
+Scaled build_scaled_csc(int m, int n, const int64_t* colptr, const int32_t* rowidx, const double* val,
+                        const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                        const double* row_up, double sign) {
+    Scaled P;
+    P.m = m, P.n = n;
+    Csr T;  // A^T (columns of the model as rows), explicit zeros left out
+    T.rows = n;
+    T.ptr.assign(size_t(n) + 1, 0);
+    int64_t keep = 0;
+    for (int64_t k = 0; k < colptr[n]; ++k) keep += val[k] != 0;
+    T.idx.reserve(size_t(keep)), T.val.reserve(size_t(keep));
+    for (int j = 0; j < n; ++j) {
+        for (int64_t k = colptr[j]; k < colptr[j + 1]; ++k)
+            if (val[k] != 0) T.idx.push_back(rowidx[k]), T.val.push_back(val[k]);
+        T.ptr[size_t(j) + 1] = int32_t(T.idx.size());
+    }
+    P.A = transpose(T, m);
+    finish_scaling(P, cost, col_lo, col_up, row_lo, row_up, sign);
+    return P;
+}
+
+Scaled build_scaled(const Model& md, double sign) {
+    const int n = int(md.cols.size()), m = int(md.row_lo.size());
+    std::vector<int64_t> colptr(static_cast<size_t>(n) + 1, 0);
+    for (int j = 0; j < n; ++j) colptr[size_t(j) + 1] = colptr[size_t(j)] + int64_t(md.cols[size_t(j)].size());
+    const size_t nz = static_cast<size_t>(colptr[size_t(n)]);
+    std::vector<int32_t> rowidx(nz);
+    std::vector<double> val(nz);
+    for (int j = 0; j < n; ++j) {
+        size_t k = size_t(colptr[size_t(j)]);
+        for (const Entry& e : md.cols[size_t(j)]) rowidx[k] = e.index, val[k] = e.value, ++k;
+    }
+    return build_scaled_csc(m, n, colptr.data(), rowidx.data(), val.data(), md.cost.data(), md.col_lo.data(),
+                            md.col_up.data(), md.row_lo.data(), md.row_up.data(), sign);
+}
+
+}  // namespace tpdhg

@@ -1,0 +1,41 @@
+// Host-side problem preparation for the PDHG path: CSR storage, transpose, and the scaled problem
+// (Ruiz + Pock-Chambolle), FP64. Semantics of gpu/pdhg.cu:48-133.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "model.hpp"
+
+namespace tpdhg {
+
+struct Csr {  // rows x cols, column indices ascending within each row
+    int rows = 0;
+    std::vector<int32_t> ptr, idx;
+    std::vector<double> val;
+    int64_t nnz() const { return int64_t(val.size()); }
+};
+
+// Counting-sort transpose; `cols` is the column count of `a` (gpu/pdhg.cu:55-70).
+Csr transpose(const Csr& a, int cols);
+
+// Scaled problem: A_s = diag(Dr) A diag(Dc); x = Dc x_s; z = Dr z_s (gpu/pdhg.cu:72-78).
+struct Scaled {
+    int m = 0, n = 0;
+    Csr A, AT;
+    std::vector<double> c, l, u, rl, ru, Dr, Dc;
+    double bnorm = 0, cnorm = 0;  // original-space norms for the relative stopping test
+};
+
+// Ruiz (10 passes, inf-norm) then Pock-Chambolle (alpha = 1) equilibration of the model's A, and the scaled
+// cost, bounds and norms; sign = -1 turns a maximisation into a minimisation (gpu/pdhg.cu:80-133).
+Scaled build_scaled(const Model& md, double sign);
+
+// Same from a column-wise model given as arrays (CSC, rows ascending per column).
+Scaled build_scaled_csc(int m, int n, const int64_t* colptr, const int32_t* rowidx, const double* val,
+                        const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                        const double* row_up, double sign);
+
+// The reference's row measure b_i: the larger finite |row bound|, 0 when both are infinite.
+double row_bound_measure(double lo, double up);
+
+}  // namespace tpdhg

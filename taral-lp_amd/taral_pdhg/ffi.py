@@ -1,0 +1,393 @@
+"""ctypes binding of libtaral_pdhg.so (include/taral_pdhg.h), the MI355X restarted-PDHG LP solver.
+
+The Python side mirrors the reference's backend concept (gpu/pdhg.cu:313-514: init, power_norm, set_steps,
+start, iterate, eval, restart, download) and its solve path (gpu/pdhg.cu:913-935), so tests read like the
+reference's own harness. All compute runs in the HIP library; this module only moves arrays across the C ABI.
+It raises if the library is missing, and the library raises (TPDHG_ERR_NO_DEVICE) when no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "build", "libtaral_pdhg.so")
+CLI_PATH = os.path.join(PKG_ROOT, "build", "pdhg")
+
+OK, ERR_ARG, ERR_PARSE, ERR_UNSUPPORTED, ERR_DEVICE, ERR_NO_DEVICE, ERR_TIME = 0, -1, -2, -3, -4, -5, -6
+STATUS = {0: "near_optimal", 1: "iteration_limit", 2: "time_limit", 3: "numerical_failure"}
+KKT_NAMES = ["PR2S", "PR2O", "PR2R", "DROW", "DZ2", "DR2S", "DR2O", "POBJ", "DCOL", "DX2"]
+
+# every symbol include/taral_pdhg.h declares
+EXPORTS = [
+    "tpdhg_last_error", "tpdhg_status_string", "tpdhg_device_count",
+    "tpdhg_model_read_mps", "tpdhg_model_read_mps_text", "tpdhg_model_from_csc", "tpdhg_model_free",
+    "tpdhg_model_get_info", "tpdhg_model_export", "tpdhg_model_name_of", "tpdhg_model_input_error",
+    "tpdhg_scaled_build", "tpdhg_scaled_view", "tpdhg_scaled_free",
+    "tpdhg_dev_create", "tpdhg_dev_power_norm", "tpdhg_dev_set_steps", "tpdhg_dev_start", "tpdhg_dev_iterate",
+    "tpdhg_dev_eval", "tpdhg_dev_eval2", "tpdhg_dev_restart", "tpdhg_dev_download", "tpdhg_dev_spmv",
+    "tpdhg_dev_info", "tpdhg_dev_set_profile", "tpdhg_dev_kernel_times", "tpdhg_dev_destroy",
+    "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
+    "tpdhg_result_json", "tpdhg_spmv_bench",
+]
+
+
+class TpdhgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_i64 = C.POINTER(C.c_int64)
+
+
+class ModelInfo(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("nnz", C.c_int64), ("maximize", C.c_int32),
+                ("has_integers", C.c_int32), ("qobj_terms", C.c_int64), ("sentinel_bounds", C.c_int64),
+                ("obj_const", C.c_double)]
+
+
+class ScaledArrays(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("nnz", C.c_int64),
+                ("a_ptr", _i32), ("a_idx", _i32), ("a_val", _d), ("at_ptr", _i32), ("at_idx", _i32), ("at_val", _d),
+                ("c", _d), ("l", _d), ("u", _d), ("rl", _d), ("ru", _d), ("Dr", _d), ("Dc", _d),
+                ("bnorm", C.c_double), ("cnorm", C.c_double)]
+
+
+class Options(C.Structure):
+    _fields_ = [("tol", C.c_double), ("time_limit", C.c_double), ("max_iter", C.c_int64),
+                ("eval_freq", C.c_int32), ("row_rel", C.c_double), ("precision_bits", C.c_int32),
+                ("refine", C.c_int32), ("inner_tol", C.c_double), ("max_refinements", C.c_int32),
+                ("use_graph", C.c_int32), ("verbose", C.c_int32), ("profile_kernels", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int64), ("restarts", C.c_int32),
+                ("refinements", C.c_int32), ("from_average", C.c_int32), ("objective", C.c_double),
+                ("rel_primal", C.c_double), ("rel_dual", C.c_double), ("rel_gap", C.c_double),
+                ("max_row_violation", C.c_double), ("max_bound_violation", C.c_double),
+                ("setup_s", C.c_double), ("parse_s", C.c_double), ("scale_s", C.c_double),
+                ("device_init_s", C.c_double), ("transfer_s", C.c_double), ("solve_s", C.c_double),
+                ("power_iter_s", C.c_double), ("total_s", C.c_double), ("norm_a", C.c_double),
+                ("primal_weight", C.c_double), ("device_mem_bytes", C.c_int64), ("m", C.c_int32), ("n", C.c_int32),
+                ("nnz", C.c_int64), ("row_kernel_ms", C.c_double), ("col_kernel_ms", C.c_double),
+                ("row_launches", C.c_int64), ("col_launches", C.c_int64), ("device", C.c_char * 64)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises FileNotFoundError when it was not built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: build it with `make -C taral-lp_amd` "
+                                    "(the solver has no Python or CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        L.tpdhg_last_error.restype = C.c_char_p
+        L.tpdhg_status_string.restype = C.c_char_p
+        L.tpdhg_model_name_of.restype = C.c_char_p
+        L.tpdhg_model_input_error.restype = C.c_char_p
+        L.tpdhg_model_read_mps.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.tpdhg_model_read_mps_text.argtypes = [C.c_char_p, C.c_int64, C.POINTER(C.c_void_p)]
+        L.tpdhg_model_from_csc.argtypes = [C.c_int32, C.c_int32, _i64, _i32, _d, _d, _d, _d, _d, _d, C.c_double,
+                                           C.c_int32, C.POINTER(C.c_void_p)]
+        L.tpdhg_model_free.argtypes = [C.c_void_p]
+        L.tpdhg_model_get_info.argtypes = [C.c_void_p, C.POINTER(ModelInfo)]
+        L.tpdhg_model_export.argtypes = [C.c_void_p, _i64, _i32, _d, _d, _d, _d, _d, _d, C.POINTER(C.c_int8)]
+        L.tpdhg_model_name_of.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.tpdhg_model_input_error.argtypes = [C.c_void_p]
+        L.tpdhg_scaled_build.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        L.tpdhg_scaled_view.argtypes = [C.c_void_p, C.POINTER(ScaledArrays)]
+        L.tpdhg_scaled_free.argtypes = [C.c_void_p]
+        L.tpdhg_dev_create.argtypes = [C.POINTER(ScaledArrays), C.c_int32, C.c_int32, C.c_int32,
+                                       C.POINTER(C.c_void_p)]
+        L.tpdhg_dev_power_norm.argtypes = [C.c_void_p, C.c_int32, _d]
+        L.tpdhg_dev_set_steps.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.tpdhg_dev_start.argtypes = [C.c_void_p, _d, _d]
+        L.tpdhg_dev_iterate.argtypes = [C.c_void_p, C.c_int32]
+        L.tpdhg_dev_eval.argtypes = [C.c_void_p, C.c_int32, C.c_double, _d]
+        L.tpdhg_dev_eval2.argtypes = [C.c_void_p, C.c_double, _d, _d]
+        L.tpdhg_dev_restart.argtypes = [C.c_void_p, C.c_int32, C.c_double]
+        L.tpdhg_dev_download.argtypes = [C.c_void_p, C.c_int32, C.c_double, _d, _d]
+        L.tpdhg_dev_spmv.argtypes = [C.c_void_p, C.c_int32, _d, _d]
+        L.tpdhg_dev_info.argtypes = [C.c_void_p, _i64, C.c_char_p, C.c_int32]
+        L.tpdhg_dev_set_profile.argtypes = [C.c_void_p, C.c_int32]
+        L.tpdhg_dev_kernel_times.argtypes = [C.c_void_p, _d, _d, _i64, _i64]
+        L.tpdhg_dev_destroy.argtypes = [C.c_void_p]
+        L.tpdhg_options_default.argtypes = [C.POINTER(Options)]
+        L.tpdhg_solve.argtypes = [C.c_void_p, C.POINTER(Options), _d, _d, C.POINTER(Result), _d, _d]
+        L.tpdhg_session_create.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(C.c_void_p)]
+        L.tpdhg_session_solve.argtypes = [C.c_void_p, _d, _d, C.POINTER(Result), _d, _d]
+        L.tpdhg_session_free.argtypes = [C.c_void_p]
+        L.tpdhg_result_json.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result), C.c_char_p, C.c_int32]
+        L.tpdhg_spmv_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d]
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != OK:
+        raise TpdhgError(code, lib().tpdhg_last_error().decode(errors="replace"))
+
+
+def _p(a, t=_d):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_count() -> int:
+    return lib().tpdhg_device_count()
+
+
+class Model:
+    """The input contract: an LP read from MPS (read_mps, src/mps.cpp:388) or built from column-wise arrays."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def from_mps(cls, path: str) -> "Model":
+        h = C.c_void_p()
+        _check(lib().tpdhg_model_read_mps(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_mps_text(cls, text: str) -> "Model":
+        b = text.encode()
+        h = C.c_void_p()
+        _check(lib().tpdhg_model_read_mps_text(b, len(b), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_csc(cls, m, n, colptr, rowidx, val, cost, col_lo, col_up, row_lo, row_up, obj_const=0.0,
+                 maximize=False) -> "Model":
+        keep = [np.ascontiguousarray(colptr, dtype=np.int64), np.ascontiguousarray(rowidx, dtype=np.int32)]
+        keep += [_f64(a) for a in (val, cost, col_lo, col_up, row_lo, row_up)]
+        h = C.c_void_p()
+        _check(lib().tpdhg_model_from_csc(int(m), int(n), _p(keep[0], _i64), _p(keep[1], _i32),
+                                          *[_p(a) for a in keep[2:]], float(obj_const), int(bool(maximize)),
+                                          C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_lp(cls, md) -> "Model":
+        """From a gen.LPModel."""
+        return cls.from_csc(md.m, md.n, md.colptr, md.rowidx, md.val, md.cost, md.col_lo, md.col_up, md.row_lo,
+                            md.row_up, md.obj_const, md.maximize)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.tpdhg_model_free(self._h)
+            self._h = C.c_void_p()
+
+    def info(self) -> ModelInfo:
+        i = ModelInfo()
+        _check(lib().tpdhg_model_get_info(self._h, C.byref(i)))
+        return i
+
+    def arrays(self) -> dict:
+        i = self.info()
+        m, n, nnz = i.m, i.n, i.nnz
+        out = dict(colptr=np.zeros(n + 1, np.int64), rowidx=np.zeros(max(nnz, 1), np.int32),
+                   val=np.zeros(max(nnz, 1)), cost=np.zeros(max(n, 1)), col_lo=np.zeros(max(n, 1)),
+                   col_up=np.zeros(max(n, 1)), row_lo=np.zeros(max(m, 1)), row_up=np.zeros(max(m, 1)),
+                   is_int=np.zeros(max(n, 1), np.int8))
+        _check(lib().tpdhg_model_export(self._h, _p(out["colptr"], _i64), _p(out["rowidx"], _i32), _p(out["val"]),
+                                        _p(out["cost"]), _p(out["col_lo"]), _p(out["col_up"]), _p(out["row_lo"]),
+                                        _p(out["row_up"]), out["is_int"].ctypes.data_as(C.POINTER(C.c_int8))))
+        for k, size in (("rowidx", nnz), ("val", nnz), ("cost", n), ("col_lo", n), ("col_up", n), ("row_lo", m),
+                        ("row_up", m), ("is_int", n)):
+            out[k] = out[k][:size]
+        return out
+
+    def name_of(self, is_row: bool, i: int) -> str:
+        return lib().tpdhg_model_name_of(self._h, int(bool(is_row)), int(i)).decode()
+
+    def input_error(self):
+        r = lib().tpdhg_model_input_error(self._h)
+        return None if r is None else r.decode()
+
+
+class Scaled:
+    """build_scaled (gpu/pdhg.cu:80-133) on the host in FP64; arrays are copied out on access."""
+
+    def __init__(self, model: Model):
+        h = C.c_void_p()
+        _check(lib().tpdhg_scaled_build(model._h, C.byref(h)))
+        self._h = h
+        self.view = ScaledArrays()
+        _check(lib().tpdhg_scaled_view(self._h, C.byref(self.view)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.tpdhg_scaled_free(self._h)
+            self._h = C.c_void_p()
+
+    def vec(self, name):
+        k = self.view.m if name in ("rl", "ru", "Dr") else self.view.n
+        return np.ctypeslib.as_array(getattr(self.view, name), shape=(k,)).copy() if k else np.zeros(0)
+
+    def csr(self, which="A"):
+        rows = self.view.m if which == "A" else self.view.n
+        pre = "a_" if which == "A" else "at_"
+        nnz = self.view.nnz
+        g = lambda k, s: np.ctypeslib.as_array(getattr(self.view, pre + k), shape=(s,)).copy()  # noqa: E731
+        return g("ptr", rows + 1), (g("idx", nnz) if nnz else np.zeros(0, np.int32)), (g("val", nnz) if nnz else np.zeros(0))
+
+
+class Device:
+    """The device backend (Gpu<T>, gpu/pdhg.cu:313-514) on the current HIP device."""
+
+    def __init__(self, scaled: Scaled, precision_bits=64, eval_freq=64, use_graph=True):
+        self.scaled = scaled  # keeps the host arrays alive
+        self.m, self.n = scaled.view.m, scaled.view.n
+        h = C.c_void_p()
+        _check(lib().tpdhg_dev_create(C.byref(scaled.view), int(precision_bits), int(eval_freq), int(bool(use_graph)),
+                                      C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.tpdhg_dev_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def power_norm(self, iters=200) -> float:
+        r = C.c_double()
+        _check(lib().tpdhg_dev_power_norm(self._h, iters, C.byref(r)))
+        return r.value
+
+    def set_steps(self, tau, sigma):
+        _check(lib().tpdhg_dev_set_steps(self._h, float(tau), float(sigma)))
+
+    def start(self, x0=None, z0=None):
+        x0, z0 = _f64(x0), _f64(z0)
+        _check(lib().tpdhg_dev_start(self._h, _p(x0), _p(z0)))
+
+    def iterate(self, steps):
+        _check(lib().tpdhg_dev_iterate(self._h, int(steps)))
+
+    def eval(self, avg=False, f=1.0):
+        out = np.zeros(10)
+        _check(lib().tpdhg_dev_eval(self._h, int(bool(avg)), float(f), _p(out)))
+        return out
+
+    def eval2(self, f=1.0):
+        cur, avg = np.zeros(10), np.zeros(10)
+        _check(lib().tpdhg_dev_eval2(self._h, float(f), _p(cur), _p(avg)))
+        return cur, avg
+
+    def restart(self, avg, f):
+        _check(lib().tpdhg_dev_restart(self._h, int(bool(avg)), float(f)))
+
+    def download(self, avg=False, f=1.0):
+        x, z = np.zeros(max(self.n, 1)), np.zeros(max(self.m, 1))
+        _check(lib().tpdhg_dev_download(self._h, int(bool(avg)), float(f), _p(x), _p(z)))
+        return x[:self.n], z[:self.m]
+
+    def spmv(self, v, transpose=False):
+        v = _f64(v)
+        y = np.zeros(max(self.n if transpose else self.m, 1))
+        _check(lib().tpdhg_dev_spmv(self._h, int(bool(transpose)), _p(v), _p(y)))
+        return y[:(self.n if transpose else self.m)]
+
+    def info(self):
+        mem = C.c_int64()
+        buf = C.create_string_buffer(128)
+        _check(lib().tpdhg_dev_info(self._h, C.byref(mem), buf, 128))
+        return dict(mem_used=mem.value, name=buf.value.decode())
+
+    def set_profile(self, on=True):
+        _check(lib().tpdhg_dev_set_profile(self._h, int(bool(on))))
+
+    def kernel_times(self):
+        a, b = C.c_double(), C.c_double()
+        na, nb = C.c_int64(), C.c_int64()
+        _check(lib().tpdhg_dev_kernel_times(self._h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb)))
+        return dict(row_ms=a.value, col_ms=b.value, row_launches=na.value, col_launches=nb.value)
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().tpdhg_options_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def _result_dict(model, o, r, x, y, m, n):
+    out = {name: getattr(r, name) for name, _ in Result._fields_ if name != "device"}
+    out["device"] = r.device.decode(errors="replace")
+    out["status"] = STATUS[r.status]
+    out["x"], out["y"] = x[:n], y[:m]
+    buf = C.create_string_buffer(8192)
+    _check(lib().tpdhg_result_json(model._h, C.byref(o), C.byref(r), buf, 8192))
+    out["json"] = buf.value.decode()
+    return out
+
+
+class Session:
+    """A problem resident on the GPU (scaled and uploaded once); each solve() is a full solve from a cold or
+    warm start."""
+
+    def __init__(self, model: Model, **opts):
+        self.model = model
+        self.opts = default_options(**opts)
+        i = model.info()
+        self.m, self.n = i.m, i.n
+        h = C.c_void_p()
+        _check(lib().tpdhg_session_create(model._h, C.byref(self.opts), C.byref(h)))
+        self._h = h
+
+    def solve(self, x0=None, y0=None) -> dict:
+        x, y = np.zeros(max(self.n, 1)), np.zeros(max(self.m, 1))
+        r = Result()
+        x0, y0 = _f64(x0), _f64(y0)
+        _check(lib().tpdhg_session_solve(self._h, _p(x0), _p(y0), C.byref(r), _p(x), _p(y)))
+        return _result_dict(self.model, self.opts, r, x, y, self.m, self.n)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.tpdhg_session_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+def solve(model: Model, x0=None, y0=None, **opts) -> dict:
+    """Scale, upload and solve on the GPU (the pdhg CLI path). Returns the result fields plus x (original space)
+    and y = Dr z (row multipliers, minimisation form)."""
+    o = default_options(**opts)
+    i = model.info()
+    x, y = np.zeros(max(i.n, 1)), np.zeros(max(i.m, 1))
+    r = Result()
+    x0, y0 = _f64(x0), _f64(y0)
+    _check(lib().tpdhg_solve(model._h, C.byref(o), _p(x0), _p(y0), C.byref(r), _p(x), _p(y)))
+    return _result_dict(model, o, r, x, y, i.m, i.n)
+
+
+def spmv_bench(ptr, idx, val, v, precision_bits=32, reps=20):
+    """y = M v with the solver's SpMV kernel; returns (y, mean_ms, median_ms) from HIP events."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    val, v = _f64(val), _f64(v)
+    rows = ptr.size - 1
+    y = np.zeros(max(rows, 1))
+    mean, med = C.c_double(), C.c_double()
+    _check(lib().tpdhg_spmv_bench(int(precision_bits), rows, v.size, _p(ptr, _i32), _p(idx, _i32), _p(val), _p(v),
+                                  _p(y), int(reps), C.byref(mean), C.byref(med)))
+    return y[:rows], mean.value, med.value

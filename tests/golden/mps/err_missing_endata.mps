@@ -1,0 +1,6 @@
+NAME E
+ROWS
+ N obj
+ L r
+COLUMNS
+ x obj 1 r 1

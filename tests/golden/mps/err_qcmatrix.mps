@@ -1,0 +1,9 @@
+NAME E
+ROWS
+ N obj
+ L r
+COLUMNS
+ x obj 1 r 1
+QCMATRIX r
+ x x 1
+ENDATA

@@ -1,0 +1,102 @@
+"""The C-ABI library and the pdhg CLI on the CPU: exported symbols, the input gate, exit codes.
+
+No compute call needs a GPU here: the gate runs before any device work, and without a device the solve calls
+report TPDHG_ERR_NO_DEVICE instead of falling back to a CPU path.
+"""
+import os
+import re
+import subprocess
+
+import pytest
+
+from taral_pdhg import ffi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MPS = os.path.join(ROOT, "tests", "golden", "mps")
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "taral_pdhg.h")).read()
+    declared = set(re.findall(r"\b(tpdhg_[a-z0-9_]+)\s*\(", header))
+    assert declared == set(ffi.EXPORTS)
+    lib = ffi.lib()
+    for name in declared:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", ffi.LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared:
+        assert re.search(rf"\bT {name}$", out, re.M), name
+
+
+GATE_REJECT = {"gate_quadobj": ("quadratic",), "gate_qmatrix": ("quadratic",), "gate_qsection": ("quadratic",),
+               "gate_maximize_qp": ("quadratic",), "gate_intorg": ("integer",), "gate_binary_bound": ("integer",),
+               "gate_integer_lower": ("integer",), "gate_integer_upper": ("integer",),
+               "gate_fixed_integer": ("integer",), "gate_miqp": ("quadratic", "integer")}
+
+
+@pytest.mark.parametrize("case", sorted(GATE_REJECT))
+@pytest.mark.parametrize("device", ["gpu", "cpu"])
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("existing", [False, True])
+def test_cli_rejects_unsupported_models(case, device, fp32, existing, tmp_path):
+    """gpu/pdhg_input_tests.py:89-109: exit 2, stdout exactly the status line, the reason on stderr, and no
+    output file created or modified."""
+    outs = [tmp_path / "a.json", tmp_path / "a.sol", tmp_path / "a.dual"]
+    for o in outs:
+        if existing:
+            o.write_text("sentinel\n")
+    cmd = [ffi.CLI_PATH, os.path.join(MPS, case + ".mps"), "--device", device, "--json", str(outs[0]), "--sol",
+           str(outs[1]), "--dual", str(outs[2])] + (["--fp32"] if fp32 else [])
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 2
+    assert r.stdout.strip() == "status unsupported_model"
+    assert "continuous LPs only" in r.stderr and all(w in r.stderr for w in GATE_REJECT[case])
+    for o in outs:
+        assert (o.read_text() == "sentinel\n") if existing else not o.exists()
+
+
+def test_c_abi_gate_and_no_device():
+    md = ffi.Model.from_mps(os.path.join(MPS, "gate_quadobj.mps"))
+    with pytest.raises(ffi.TpdhgError) as e:
+        ffi.solve(md)
+    assert e.value.code == ffi.ERR_UNSUPPORTED and "quadratic" in str(e.value)
+    if ffi.device_count() == 0:
+        ok = ffi.Model.from_mps(os.path.join(MPS, "gate_continuous_lp.mps"))
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.solve(ok)
+        assert e.value.code == ffi.ERR_NO_DEVICE
+
+
+def test_cli_parse_error_and_usage(tmp_path):
+    r = subprocess.run([ffi.CLI_PATH, os.path.join(MPS, "err_bad_number.mps")], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stdout.startswith("status parse_error: bad number '1e'")
+    r = subprocess.run([ffi.CLI_PATH], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([ffi.CLI_PATH, os.path.join(MPS, "gate_continuous_lp.mps"), "--device", "tpu"],
+                       capture_output=True, text=True)
+    assert r.returncode == 2
+    # --device cpu on a supported model: refused (this build has no CPU backend), no file written
+    out = tmp_path / "x.json"
+    r = subprocess.run([ffi.CLI_PATH, os.path.join(MPS, "gate_continuous_lp.mps"), "--device", "cpu", "--json",
+                        str(out)], capture_output=True, text=True)
+    assert r.returncode == 2 and "GPU only" in r.stderr and not out.exists()
+
+
+def test_model_from_csc_validates():
+    import numpy as np
+    good = dict(m=2, n=2, colptr=[0, 1, 2], rowidx=[0, 1], val=[1.0, 2.0], cost=[1.0, 1.0], col_lo=[0, 0],
+                col_up=[1, 1], row_lo=[0, 0], row_up=[1, 1])
+    assert ffi.Model.from_csc(**good).info().nnz == 2
+    for bad in (dict(rowidx=[0, 2]), dict(colptr=[0, 2, 1]), dict(val=[1.0, np.nan])):
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.Model.from_csc(**{**good, **bad})
+        assert e.value.code == ffi.ERR_ARG
+
+
+def test_session_refuses_without_device_or_bad_options():
+    md = ffi.Model.from_mps(os.path.join(MPS, "gate_continuous_lp.mps"))
+    with pytest.raises(ffi.TpdhgError) as e:
+        ffi.Session(md, precision_bits=16)
+    assert e.value.code == ffi.ERR_ARG
+    with pytest.raises(ffi.TpdhgError) as e:
+        ffi.Session(md, precision_bits=64, refine=1)
+    assert e.value.code == ffi.ERR_ARG
