@@ -3,6 +3,7 @@
 #include "pdhg_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -11,6 +12,15 @@ static double or_now(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Progress trace to stderr every N evaluations (0 = off), from the ORACLE_TRACE environment variable. */
+static int or_trace_every = -1;
+static void or_trace_init(void) {
+    if (or_trace_every < 0) {
+        const char* e = getenv("ORACLE_TRACE");
+        or_trace_every = e ? atoi(e) : 0;
+    }
 }
 
 /* Deterministic power-iteration start, gpu/pdhg.cu:153 */
@@ -200,6 +210,7 @@ int or_solve(const or_scaled* P, int fp32, double tol, long max_iter, int eval_f
              double time_limit, const double* x0, const double* z0, double* x_out, double* z_out,
              or_result* res) {
     if (eval_freq < 1) eval_freq = 1;
+    or_trace_init();
     return fp32 ? or_run_f(P, tol, max_iter, eval_freq, row_rel, time_limit, x0, z0, x_out, z_out, res)
                 : or_run_d(P, tol, max_iter, eval_freq, row_rel, time_limit, x0, z0, x_out, z_out, res);
 }
