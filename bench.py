@@ -82,7 +82,11 @@ def main():
     ap.add_argument("--mode", default="auto", choices=["auto", "fp64", "fp32", "fp32-refine"])
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--time-limit", type=float, default=600.0)
+    ap.add_argument("--max-iter", type=int, default=2_000_000_000,
+                    help="iteration cap per solve (profiling runs; the reported value then is not a full solve)")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernels directly instead of replaying HIP graphs (used under rocprofv3)")
     ap.add_argument("--cpu-sample-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
@@ -100,7 +104,8 @@ def main():
     print(f"[bench] generated {m}x{n} nnz={md.nnz} in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
 
     def timed(mode_name, steps, warmup, profile):
-        opts = dict(tol=a.tol, time_limit=a.time_limit, profile_kernels=int(profile), **mode_options(mode_name))
+        opts = dict(tol=a.tol, time_limit=a.time_limit, max_iter=a.max_iter, profile_kernels=int(profile),
+                    use_graph=0 if a.no_graph else 1, **mode_options(mode_name))
         t0 = time.perf_counter()
         sess = ffi.Session(model, **opts)
         setup = time.perf_counter() - t0
