@@ -45,14 +45,27 @@ def kernel_bytes(m, n, nnz, s):
     return row, col
 
 
+MODES = {  # name: solver options
+    "fast": dict(precision_bits=32, halpern=1, refine=1),   # FP32 reflected Halpern + FP64 refinement
+    "fp64": dict(precision_bits=64),                        # the reference algorithm, plain FP64 cold start
+    "fp32": dict(precision_bits=32),                        # the reference algorithm with FP32 iterates
+    "halpern-fp64": dict(precision_bits=64, halpern=1),
+}
+
+
+def halpern_kernel_bytes(m, n, nnz, s):
+    """Same for the fast path's kernels: row kernel reads w once (s*n) and per row z, zh, rl, ru, z0 and writes z
+    (6 s per row); column kernel reads z once (s*m) and per column x, x+, x0, c, l, u and writes x, x+, w
+    (9 s per column; the A^T z store of a block's last step is not counted)."""
+    row = (s + 4) * nnz + 4 * (m + 1) + s * n + 6 * s * m
+    col = (s + 4) * nnz + 4 * (n + 1) + s * m + 9 * s * n
+    return row, col
+
+
 def mode_options(mode):
-    if mode == "fp64":
-        return dict(precision_bits=64)
-    if mode == "fp32":
-        return dict(precision_bits=32)
-    if mode == "fp32-refine":
-        return dict(precision_bits=32, refine=1)
-    raise SystemExit(f"unknown mode {mode}")
+    if mode not in MODES:
+        raise SystemExit(f"unknown mode {mode}")
+    return dict(MODES[mode])
 
 
 def cpu_baseline(md, iterations, sample_iters):
@@ -79,12 +92,13 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg1", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="auto", choices=["auto", "fp64", "fp32", "fp32-refine"])
+    ap.add_argument("--mode", default="fast", choices=sorted(MODES))
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--time-limit", type=float, default=600.0)
     ap.add_argument("--max-iter", type=int, default=2_000_000_000,
                     help="iteration cap per solve (profiling runs; the reported value then is not a full solve)")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="solver progress on stderr every 50 evaluations")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying HIP graphs (used under rocprofv3)")
     ap.add_argument("--cpu-sample-iters", type=int, default=3)
@@ -94,7 +108,7 @@ def main():
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a design only "
               "(see the sharding section of the design notes)", file=sys.stderr)
         sys.exit(2)
-    mode = "fp64" if a.mode == "auto" else a.mode
+    mode = a.mode
 
     from taral_pdhg import ffi, gen
     m, n, k, seed = WORKLOADS[a.workload]
@@ -105,7 +119,7 @@ def main():
 
     def timed(mode_name, steps, warmup, profile):
         opts = dict(tol=a.tol, time_limit=a.time_limit, max_iter=a.max_iter, profile_kernels=int(profile),
-                    use_graph=0 if a.no_graph else 1, **mode_options(mode_name))
+                    use_graph=0 if a.no_graph else 1, verbose=int(a.verbose), **mode_options(mode_name))
         t0 = time.perf_counter()
         sess = ffi.Session(model, **opts)
         setup = time.perf_counter() - t0
@@ -128,8 +142,8 @@ def main():
     wall, res, setup = timed(mode, a.steps, a.warmup, profile=True)
     last = res[-1]
     value = a.steps / wall
-    s_val = 8 if mode == "fp64" else 4
-    b_row, b_col = kernel_bytes(m, n, last["nnz"], s_val)
+    s_val = 8 if MODES[mode]["precision_bits"] == 64 else 4
+    b_row, b_col = (halpern_kernel_bytes if MODES[mode].get("halpern") else kernel_bytes)(m, n, last["nnz"], s_val)
     row_ms = sum(r["row_kernel_ms"] for r in res)
     col_ms = sum(r["col_kernel_ms"] for r in res)
     row_n = sum(r["row_launches"] for r in res)
@@ -152,7 +166,8 @@ def main():
         "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 (f64 reductions)", "fp32-refine": "f32 iterates + f64 refinement"}[mode],
+        "dtype": {"fp64": "f64", "fp32": "f32 (f64 reductions)", "fast": "f32 iterates + f64 refinement",
+                  "halpern-fp64": "f64"}[mode],
         "data": "synthetic",
         "config": {"workload": f"planted equality LP m={m} n={n} nnz/col={k} seed={seed}, 0<=x<=10, tol {a.tol:g}",
                    "m": m, "n": n, "nnz": last["nnz"], "tol": a.tol, "mode": mode, "parallelism": "single GPU"},

@@ -120,6 +120,33 @@ int tpdhg_dev_kernel_times(tpdhg_dev* dev, double* row_ms, double* col_ms, int64
                            int64_t* col_launches);
 void tpdhg_dev_destroy(tpdhg_dev* dev);
 
+/* ------------------------------------------------------------------ fast-path backend (no reference counterpart) */
+/* Reflected Halpern PDHG on a problem recentred around an FP64 base point (x_b, z_b), scaled space: the inner LP
+ * is  min (c + A^T z_b)' dx  s.t.  rl - A x_b <= A dx <= ru - A x_b,  l - x_b <= dx <= u - x_b. One iteration k
+ * with anchors (x0, z0), lambda = (k+1)/(k+2) and reflection rho: x+ = proj(x - tau (c_in + A^T z)),
+ * z+ = prox(z + sigma A (2 x+ - x)); x <- lambda (rho x+ - (rho-1) x) + (1-lambda) x0, and the same for z.
+ * This is the engine of tpdhg_options.halpern; these calls expose it for tests and custom drivers. */
+typedef struct tpdhg_fast tpdhg_fast;
+int tpdhg_fast_create(const tpdhg_scaled_arrays* problem, int32_t precision_bits, int32_t eval_freq, int32_t use_graph,
+                      double reflection, tpdhg_fast** out);
+int tpdhg_fast_power_norm(tpdhg_fast* f, int32_t iters, double* norm);
+/* Base point (n and m doubles, either may be NULL = 0); used from the next tpdhg_fast_begin_round. */
+int tpdhg_fast_set_base(tpdhg_fast* f, const double* x_b, const double* z_b);
+/* Inner problem from the base; inner iterates reset to dx = clamp(0, l_in, u_in), dz = 0. */
+int tpdhg_fast_begin_round(tpdhg_fast* f);
+int tpdhg_fast_set_steps(tpdhg_fast* f, double tau, double sigma);
+/* Anchors = current inner point, iteration counter 0, first primal step. */
+int tpdhg_fast_restart(tpdhg_fast* f);
+int tpdhg_fast_iterate(tpdhg_fast* f, int32_t steps);
+/* KKT scalars (slots as tpdhg_dev_eval) of base + inner on the original scaled problem, FP64; DX2/DZ2 measure
+ * the distance from the anchors. */
+int tpdhg_fast_eval(tpdhg_fast* f, double out[10]);
+/* base += inner point. */
+int tpdhg_fast_end_round(tpdhg_fast* f);
+/* base + inner point (FP64, scaled space); either may be NULL. */
+int tpdhg_fast_download(tpdhg_fast* f, double* x, double* z);
+void tpdhg_fast_destroy(tpdhg_fast* f);
+
 /* ------------------------------------------------------------------ full solve (the pdhg CLI path) */
 typedef struct {
     double tol;        /* relative KKT tolerance (default 1e-4, gpu/pdhg.cu:745) */
@@ -128,12 +155,15 @@ typedef struct {
     int32_t eval_freq; /* iterations per evaluation (default 64) */
     double row_rel;    /* per-row relative primal test factor, 0 = off (gpu/pdhg.cu:748) */
     int32_t precision_bits; /* 64 (default) or 32 */
-    int32_t refine;    /* 1: FP64 iterative refinement around FP32 inner solves (precision_bits 32) */
+    int32_t refine;    /* 1: FP64 iterative refinement rounds around the inner solves (needs halpern = 1) */
     double inner_tol;  /* relative KKT target of each refinement round (default 1e-4) */
     int32_t max_refinements;
     int32_t use_graph; /* capture each eval_freq-iteration block in a HIP graph (default 1) */
     int32_t verbose;
     int32_t profile_kernels; /* HIP-event timing of the row/column kernels */
+    int32_t halpern;   /* 1: fast path, reflected Halpern PDHG on an FP64-recentred problem (no reference
+                          counterpart; the reference algorithm is the default, halpern = 0) */
+    double reflection; /* Halpern reflection coefficient in [1, 2) (default 1.9) */
 } tpdhg_options;
 void tpdhg_options_default(tpdhg_options* o);
 

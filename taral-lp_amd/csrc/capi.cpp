@@ -19,6 +19,9 @@ struct tpdhg_scaled {
 struct tpdhg_dev {
     std::unique_ptr<tpdhg::Device> d;
 };
+struct tpdhg_fast {
+    std::unique_ptr<tpdhg::FastDevice> d;
+};
 struct tpdhg_session {
     tpdhg::LP lp;  // the session's own copy; Session keeps a reference to it
     std::unique_ptr<tpdhg::Session> s;
@@ -66,6 +69,7 @@ tpdhg::Options to_options(const tpdhg_options* o) {
     r.row_rel = o->row_rel, r.precision_bits = o->precision_bits, r.refine = o->refine != 0;
     r.inner_tol = o->inner_tol, r.max_refinements = o->max_refinements, r.graph = o->use_graph != 0;
     r.verbose = o->verbose != 0, r.profile_kernels = o->profile_kernels != 0;
+    r.halpern = o->halpern != 0, r.rho = o->reflection;
     return r;
 }
 }  // namespace
@@ -253,6 +257,47 @@ int tpdhg_dev_kernel_times(tpdhg_dev* dev, double* row_ms, double* col_ms, int64
 }
 void tpdhg_dev_destroy(tpdhg_dev* dev) { delete dev; }
 
+int tpdhg_fast_create(const tpdhg_scaled_arrays* p, int32_t precision_bits, int32_t eval_freq, int32_t use_graph,
+                      double reflection, tpdhg_fast** out) {
+    if (!p || !out || p->m < 0 || p->n < 0 || !p->a_ptr || !p->at_ptr) return fail(TPDHG_ERR_ARG, "bad problem");
+    return guarded([&]() -> int {
+        tpdhg::Scaled S;  // owned copy of the caller's arrays
+        S.m = p->m, S.n = p->n;
+        const size_t m = size_t(p->m), n = size_t(p->n), z = size_t(p->nnz);
+        if (p->a_ptr[p->m] != p->nnz || p->at_ptr[p->n] != p->nnz) return fail(TPDHG_ERR_ARG, "CSR pointers do not match nnz");
+        S.A.rows = p->m, S.A.ptr.assign(p->a_ptr, p->a_ptr + m + 1), S.A.idx.assign(p->a_idx, p->a_idx + z);
+        S.A.val.assign(p->a_val, p->a_val + z);
+        S.AT.rows = p->n, S.AT.ptr.assign(p->at_ptr, p->at_ptr + n + 1), S.AT.idx.assign(p->at_idx, p->at_idx + z);
+        S.AT.val.assign(p->at_val, p->at_val + z);
+        S.c.assign(p->c, p->c + n), S.l.assign(p->l, p->l + n), S.u.assign(p->u, p->u + n);
+        S.rl.assign(p->rl, p->rl + m), S.ru.assign(p->ru, p->ru + m), S.Dr.assign(p->Dr, p->Dr + m);
+        S.Dc.assign(p->Dc, p->Dc + n), S.bnorm = p->bnorm, S.cnorm = p->cnorm;
+        tpdhg::FastOptions o;
+        o.precision_bits = precision_bits, o.eval_freq = eval_freq, o.use_graph = use_graph != 0, o.rho = reflection;
+        auto f = std::make_unique<tpdhg_fast>();
+        f->d = tpdhg::make_fast_device(S, o);
+        *out = f.release();
+        return TPDHG_OK;
+    });
+}
+
+#define FAST_CALL(f, body)                                                  \
+    do {                                                                    \
+        if (!(f) || !(f)->d) return fail(TPDHG_ERR_ARG, "null fast handle"); \
+        return guarded([&]() -> int { body; return TPDHG_OK; });            \
+    } while (0)
+
+int tpdhg_fast_power_norm(tpdhg_fast* f, int32_t iters, double* norm) { FAST_CALL(f, *norm = f->d->power_norm(iters)); }
+int tpdhg_fast_set_base(tpdhg_fast* f, const double* x_b, const double* z_b) { FAST_CALL(f, f->d->set_base(x_b, z_b)); }
+int tpdhg_fast_begin_round(tpdhg_fast* f) { FAST_CALL(f, f->d->begin_round()); }
+int tpdhg_fast_set_steps(tpdhg_fast* f, double tau, double sigma) { FAST_CALL(f, f->d->set_steps(tau, sigma)); }
+int tpdhg_fast_restart(tpdhg_fast* f) { FAST_CALL(f, f->d->restart()); }
+int tpdhg_fast_iterate(tpdhg_fast* f, int32_t steps) { FAST_CALL(f, f->d->iterate(steps)); }
+int tpdhg_fast_eval(tpdhg_fast* f, double out[10]) { FAST_CALL(f, f->d->eval(out)); }
+int tpdhg_fast_end_round(tpdhg_fast* f) { FAST_CALL(f, f->d->end_round()); }
+int tpdhg_fast_download(tpdhg_fast* f, double* x, double* z) { FAST_CALL(f, f->d->download(x, z)); }
+void tpdhg_fast_destroy(tpdhg_fast* f) { delete f; }
+
 void tpdhg_options_default(tpdhg_options* o) {
     if (!o) return;
     tpdhg::Options d;
@@ -260,6 +305,7 @@ void tpdhg_options_default(tpdhg_options* o) {
     o->row_rel = d.row_rel, o->precision_bits = d.precision_bits, o->refine = d.refine ? 1 : 0;
     o->inner_tol = d.inner_tol, o->max_refinements = d.max_refinements, o->use_graph = d.graph ? 1 : 0;
     o->verbose = 0, o->profile_kernels = 0;
+    o->halpern = d.halpern ? 1 : 0, o->reflection = d.rho;
 }
 
 namespace {
@@ -286,7 +332,8 @@ void fill_result(const tpdhg::Report& r, const tpdhg::LP& lp, tpdhg_result* res,
 
 int check_options(const tpdhg::Options& o) {
     if (o.precision_bits != 32 && o.precision_bits != 64) return fail(TPDHG_ERR_ARG, "precision_bits must be 32 or 64");
-    if (o.refine && o.precision_bits != 32) return fail(TPDHG_ERR_ARG, "refine needs precision_bits 32");
+    if (o.refine && !o.halpern) return fail(TPDHG_ERR_ARG, "refine runs on the Halpern path: set halpern = 1");
+    if (o.halpern && !(o.rho >= 1 && o.rho < 2)) return fail(TPDHG_ERR_ARG, "reflection must be in [1, 2)");
     return TPDHG_OK;
 }
 }  // namespace

@@ -4,9 +4,10 @@
 // GPU only: `--device cpu` is accepted by the parser (so scripts keep working) but ends with exit 2 and a
 // message, after the input gate, instead of running a CPU solver.
 //
-// Additional flags: --refine (FP64 iterative refinement around FP32 iterates; implies --fp32),
-// --inner-tol T, --max-refinements N, --profile-kernels, --warm-sol FILE / --warm-dual FILE (start point in
-// the .sol/.dual formats this program writes).
+// Additional flags: --fast (the fast path: FP32 reflected Halpern PDHG iterates inside FP64 iterative
+// refinement = --fp32 --halpern --refine), --halpern, --reflection R, --refine, --inner-tol T,
+// --max-refinements N, --profile-kernels, --warm-sol FILE / --warm-dual FILE (start point in the .sol/.dual
+// formats this program writes).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -70,7 +71,10 @@ int main(int argc, char** argv) {
         else if (a == "--fp32") o.precision_bits = 32;
         else if (a == "--no-graph") o.graph = false;
         else if (a == "--verbose") o.verbose = true;
-        else if (a == "--refine") o.refine = true, o.precision_bits = 32;
+        else if (a == "--refine") o.refine = true, o.halpern = true;
+        else if (a == "--halpern") o.halpern = true;
+        else if (a == "--reflection") o.rho = std::atof(next());
+        else if (a == "--fast") o.halpern = true, o.refine = true, o.precision_bits = 32;
         else if (a == "--inner-tol") o.inner_tol = std::atof(next());
         else if (a == "--max-refinements") o.max_refinements = std::atoi(next());
         else if (a == "--profile-kernels") o.profile_kernels = true;
@@ -82,7 +86,8 @@ int main(int argc, char** argv) {
         std::fprintf(stderr,
                      "usage: pdhg MODEL.mps --device gpu|cpu [--threads N] [--tol 1e-4] [--time-limit S] "
                      "[--max-iter N] [--json OUT.json] [--sol OUT.sol] [--dual OUT.dual] [--fp32] [--eval-freq 64] "
-                     "[--row-rel 0] [--no-graph] [--refine] [--inner-tol 1e-4] [--max-refinements 8] "
+                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--refine] [--inner-tol 1e-4] "
+                     "[--max-refinements 8] "
                      "[--profile-kernels] [--warm-sol X.sol] [--warm-dual X.dual]\n");
         return 2;
     }
@@ -103,6 +108,7 @@ int main(int argc, char** argv) {
     }
     tm.parse = since(t_start);
     (void)threads;
+    if (o.refine && !o.halpern) o.halpern = true;
     if (device == "cpu") {
         std::fprintf(stderr, "this build solves on the GPU only; use --device gpu (there is no CPU backend)\n");
         return 2;

@@ -15,6 +15,8 @@
 // are deterministic run to run. The reference computes these in gpu/pdhg.cu:236-291 with one pass per point.
 #include <hip/hip_runtime.h>
 
+#include "hipcommon.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -26,35 +28,6 @@
 
 namespace tpdhg {
 namespace {
-
-constexpr int kWG = 256;    // threads per workgroup: 4 waves of 64
-constexpr int kCap = 2048;  // nonzeros staged per row block (8 per thread)
-constexpr int kMaxRedBlocks = 1024;
-
-#define HIP_OK(call)                                                                                      \
-    do {                                                                                                  \
-        hipError_t e_ = (call);                                                                           \
-        if (e_ != hipSuccess)                                                                             \
-            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #call ")"); \
-    } while (0)
-
-// ------------------------------------------------------------------------------------------------ kernels
-__device__ __forceinline__ int xcd_block(int b, int nblk) {
-    const int q = nblk >> 3, r = nblk & 7, x = b & 7, i = b >> 3;
-    return x * q + (x < r ? x : r) + i;
-}
-
-template <class T>
-__device__ __forceinline__ T clampv(T v, T lo, T hi) {
-    return v < lo ? lo : (v > hi ? hi : v);
-}
-
-template <class T>
-struct EpiStore {  // y = M v
-    T* y;
-    __device__ void load() {}
-    __device__ void operator()(int r, T s) const { y[r] = s; }
-};
 
 // Dual (row) step, the semantics of gpu/pdhg.cu:179-190: s = (A x)_i, v = z + sigma (2 s - Ax_old),
 // z+ = v - sigma proj_[rl, ru](v / sigma); Ax = s; running sums Axs += s, zs += z+.
@@ -90,45 +63,6 @@ struct EpiCol {
     }
 };
 
-template <class T, class Epi>
-__global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restrict__ blk, const int* __restrict__ ptr,
-                                                   const int* __restrict__ idx, const T* __restrict__ val,
-                                                   const T* __restrict__ v, Epi epi) {
-    __shared__ T sh[kCap];
-    __shared__ T red[kWG / 64];
-    const int b = xcd_block(int(blockIdx.x), nblk);
-    const int tid = int(threadIdx.x);
-    const int r0 = blk[b], r1 = blk[b + 1];
-    const int k0 = ptr[r0], k1 = ptr[r1];
-    epi.load();
-    if (k1 - k0 > kCap) {  // a single long row: strided partial sums and a workgroup reduction
-        T s = 0;
-        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * v[idx[k]];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if ((tid & 63) == 0) red[tid >> 6] = s;
-        __syncthreads();
-        if (tid == 0) epi(r0, (red[0] + red[1]) + (red[2] + red[3]));
-        return;
-    }
-#pragma unroll
-    for (int u = 0; u < kCap / kWG; ++u) {
-        const int k = k0 + u * kWG + tid;
-        if (k < k1) sh[k - k0] = val[k] * v[idx[k]];
-    }
-    __syncthreads();
-    const int nr = r1 - r0;
-    int lg = 6;  // lanes per row = 2^lg: the widest group with nr groups fitting the workgroup
-    while (lg > 0 && (nr << lg) > kWG) --lg;
-    const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
-    T s = 0;
-    if (row < nr) {
-        const int a = ptr[r0 + row] - k0, e = ptr[r0 + row + 1] - k0;
-        for (int k = a + lane; k < e; k += lpr) s += sh[k];
-    }
-    for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (row < nr && lane == 0) epi(r0 + row, s);
-}
-
 // First primal step of a block from the stored A^T z (gpu/pdhg.cu:208-214).
 template <class T>
 __global__ __launch_bounds__(kWG) void k_xupd(int n, T* __restrict__ x, const T* __restrict__ ATz,
@@ -140,39 +74,6 @@ __global__ __launch_bounds__(kWG) void k_xupd(int n, T* __restrict__ x, const T*
         const T xn = clampv(x[j] - tau * (c[j] + ATz[j]), l[j], u[j]);
         x[j] = xn, xs[j] += xn;
     }
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-    return v;
-}
-
-// Workgroup sum of K doubles per thread into part[blockIdx * K + k] (lane order fixed: deterministic).
-template <int K>
-__device__ __forceinline__ void block_store(double (&v)[K], double* __restrict__ part) {
-    __shared__ double sh[K][kWG / 64];
-    const int lane = int(threadIdx.x) & 63, w = int(threadIdx.x) >> 6;
-    for (int k = 0; k < K; ++k) {
-        const double s = wave_sum(v[k]);
-        if (lane == 0) sh[k][w] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < K) {
-        const int k = int(threadIdx.x);
-        double s = 0;
-        for (int q = 0; q < kWG / 64; ++q) s += sh[k][q];
-        part[size_t(blockIdx.x) * K + k] = s;
-    }
-}
-
-__device__ __forceinline__ double row_rel_weight(double lo, double up, double dr) {
-    double b = 0;
-    if (lo > -INFINITY) b = lo < 0 ? -lo : lo;
-    if (up < INFINITY) {
-        const double a = up < 0 ? -up : up;
-        b = a > b ? a : b;
-    }
-    return 1.0 / (1.0 + b / dr);
 }
 
 template <class T>
@@ -230,37 +131,6 @@ __global__ __launch_bounds__(kWG) void k_eval2(int m, int n, int nbr, double f, 
     block_store<10>(v, part);
 }
 
-// Fixed-order final sums: out[0..9] = current KKT, out[10..19] = average KKT.
-__global__ void k_eval2_finish(int nbr, int nbc, const double* __restrict__ part, double* __restrict__ out) {
-    const int k = int(threadIdx.x);
-    if (k >= 10) return;
-    double sr = 0, sc = 0;
-    for (int q = 0; q < nbr; ++q) sr += part[size_t(q) * 10 + k];
-    for (int q = nbr; q < nbr + nbc; ++q) sc += part[size_t(q) * 10 + k];
-    const int h = k / 5, s = k % 5;
-    out[10 * h + s] = sr, out[10 * h + 5 + s] = sc;
-}
-
-template <class T>
-__global__ __launch_bounds__(kWG) void k_sumsq(int n, const T* __restrict__ x, double* __restrict__ part) {
-    double v[1] = {0};
-    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
-        v[0] += double(x[j]) * double(x[j]);
-    block_store<1>(v, part);
-}
-
-__global__ void k_sum_finish(int blocks, const double* __restrict__ part, double* __restrict__ out) {
-    if (threadIdx.x != 0) return;
-    double s = 0;
-    for (int q = 0; q < blocks; ++q) s += part[q];
-    out[0] = s;
-}
-
-template <class T>
-__global__ __launch_bounds__(kWG) void k_scale(int n, T* __restrict__ x, double f) {
-    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
-        x[j] = T(x[j] * f);
-}
 
 // Restart, gpu/pdhg.cu:293-299: x <- average (xs f) or current; xlast = x; clear both running sums.
 template <class T>
@@ -272,102 +142,7 @@ __global__ __launch_bounds__(kWG) void k_restart(int n, int avg, double f, T* __
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(kWG) void k_to_f64(int n, double f, const T* __restrict__ src, double* __restrict__ dst) {
-    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
-        dst[j] = double(src[j]) * f;
-}
-
-template <class T>
-__global__ __launch_bounds__(kWG) void k_from_f64(int n, const double* __restrict__ src, T* __restrict__ dst) {
-    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
-        dst[j] = T(src[j]);
-}
-
 // ------------------------------------------------------------------------------------------------ host side
-// Row blocks: maximal runs of rows holding at most kCap nonzeros and at most kWG rows; a row longer than
-// kCap forms a block of its own.
-std::vector<int32_t> row_blocks(int rows, const int32_t* ptr) {
-    std::vector<int32_t> b;
-    b.push_back(0);
-    int r0 = 0;
-    while (r0 < rows) {
-        int r = r0;
-        while (r < rows && r - r0 < kWG && int64_t(ptr[r + 1]) - ptr[r0] <= kCap) ++r;
-        if (r == r0) r = r0 + 1;  // long row
-        b.push_back(r);
-        r0 = r;
-    }
-    return b;
-}
-
-int grid_for(int k) { return std::max(1, std::min(2048, (k + kWG - 1) / kWG)); }
-int red_blocks(int k) { return k > 0 ? std::max(1, std::min(kMaxRedBlocks, (k + kWG - 1) / kWG)) : 0; }
-
-struct Mem {  // device allocations owned by one backend
-    std::vector<void*> ptrs;
-    template <class U>
-    U* alloc(size_t count) {
-        void* p = nullptr;
-        HIP_OK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(U)));
-        ptrs.push_back(p);
-        return static_cast<U*>(p);
-    }
-    ~Mem() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-};
-
-template <class T>
-struct DevMat {
-    int rows = 0, nblk = 0;
-    int *ptr = nullptr, *idx = nullptr, *blk = nullptr;
-    T* val = nullptr;
-};
-
-template <class T>
-DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, const int32_t* ptr, const int32_t* idx, const double* val) {
-    DevMat<T> d;
-    d.rows = rows;
-    const int64_t nnz = ptr[rows];
-    std::vector<int32_t> blk = row_blocks(rows, ptr);
-    d.nblk = int(blk.size()) - 1;
-    d.ptr = mem.alloc<int>(size_t(rows) + 1), d.idx = mem.alloc<int>(size_t(nnz)), d.blk = mem.alloc<int>(blk.size());
-    d.val = mem.alloc<T>(size_t(nnz));
-    HIP_OK(hipMemcpyAsync(d.ptr, ptr, (size_t(rows) + 1) * sizeof(int), hipMemcpyHostToDevice, st));
-    if (nnz) HIP_OK(hipMemcpyAsync(d.idx, idx, size_t(nnz) * sizeof(int), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(d.blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    if (nnz) {
-        std::vector<T> h(val, val + nnz);
-        HIP_OK(hipMemcpyAsync(d.val, h.data(), size_t(nnz) * sizeof(T), hipMemcpyHostToDevice, st));
-        HIP_OK(hipStreamSynchronize(st));
-    }
-    HIP_OK(hipStreamSynchronize(st));
-    return d;
-}
-
-template <class T, class Epi>
-void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
-    if (a.nblk == 0) return;
-    k_rowblocks<T, Epi><<<a.nblk, kWG, 0, st>>>(a.nblk, a.blk, a.ptr, a.idx, a.val, v, epi);
-}
-
-std::string device_name() {
-    int dev = 0;
-    HIP_OK(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    HIP_OK(hipGetDeviceProperties(&prop, dev));
-    std::string nm = prop.name;
-    if (nm.empty()) nm = prop.gcnArchName;
-    return nm;
-}
-
-void require_device() {
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
-        throw std::runtime_error("no HIP device visible: the PDHG solver runs on an AMD GPU only (there is no CPU path)");
-}
-
 template <class T>
 class Dev final : public Device {
 public:
@@ -480,15 +255,17 @@ public:
         if (nbr_ + nbc_ > 0) {
             EvalArgs<T> a{Ax, z, Axs, zs, rl, ru, Dr, zlast, x, ATz, xs, ATzs, c, l, u, Dc, xlast};
             k_eval2<T><<<nbr_ + nbc_, kWG, 0, st_>>>(m_, n_, nbr_, f, a, part_);
-            k_eval2_finish<<<1, 64, 0, st_>>>(nbr_, nbc_, part_, out_);
+            k_finish<<<1, kWG, 0, st_>>>(0, nbr_, 10, part_, out_);
+            k_finish<<<1, kWG, 0, st_>>>(nbr_, nbc_, 10, part_, out_ + 10);
             HIP_OK(hipMemcpyAsync(hout_, out_, 20 * sizeof(double), hipMemcpyDeviceToHost, st_));
             HIP_OK(hipStreamSynchronize(st_));
         } else {
             std::fill(hout_, hout_ + 20, 0.0);
         }
         HIP_OK(hipGetLastError());
-        if (cur) std::copy(hout_, hout_ + 10, cur);
-        if (avg) std::copy(hout_ + 10, hout_ + 20, avg);
+        // row partials: current PR2S..DZ2 then average; column partials: current DR2S..DX2 then average
+        if (cur) std::copy(hout_, hout_ + 5, cur), std::copy(hout_ + 10, hout_ + 15, cur + 5);
+        if (avg) std::copy(hout_ + 5, hout_ + 10, avg), std::copy(hout_ + 15, hout_ + 20, avg + 5);
         collect_events();
     }
 
@@ -568,7 +345,7 @@ private:
         const int b = red_blocks(k);
         if (!b) return 0;
         k_sumsq<T><<<b, kWG, 0, st_>>>(k, v, part_);
-        k_sum_finish<<<1, 64, 0, st_>>>(b, part_, out_);
+        k_finish<<<1, kWG, 0, st_>>>(0, b, 1, part_, out_);
         HIP_OK(hipMemcpyAsync(hout_, out_, sizeof(double), hipMemcpyDeviceToHost, st_));
         HIP_OK(hipStreamSynchronize(st_));
         return hout_[0];

@@ -67,6 +67,48 @@ std::unique_ptr<Device> make_device_raw(int m, int n, int64_t nnz, const int32_t
                                         const double* rl, const double* ru, const double* Dr, const double* Dc,
                                         const DeviceOptions& opt);
 
+// Backend of the fast path: reflected Halpern PDHG iterations (FP32 or FP64 iterates) on a recentred problem
+// around an FP64 base point (x_b, z_b), which is how FP64 iterative refinement and warm starts enter. In inner
+// coordinates the LP is  min (c + A^T z_b)' dx  s.t.  rl - A x_b <= A dx <= ru - A x_b,  l - x_b <= dx <= u - x_b,
+// with the row prox shifted by z_b so that inequality rows stay exact. One iteration k (anchor x0, z0,
+// lambda = (k+1)/(k+2), reflection rho):
+//   x+ = proj(x - tau (c_in + A^T z)),  z+ = prox(z + sigma A (2 x+ - x)),
+//   x <- lambda (rho x+ - (rho-1) x) + (1 - lambda) x0,  z <- same with z+, z0.
+// KKT scalars are evaluated in FP64 at the combined point base + inner on the original scaled problem, with the
+// same ten slots as Device::eval2 (DX2/DZ2 measured from the anchors).
+class FastDevice {
+public:
+    virtual ~FastDevice() = default;
+    virtual double power_norm(int iters) = 0;
+    // FP64 base point in scaled space (null = 0 for that part); takes effect at the next begin_round().
+    virtual void set_base(const double* x_b, const double* z_b) = 0;
+    // Inner problem from the current base; inner iterates reset to dx = clamp(0, l_in, u_in), dz = 0.
+    virtual void begin_round() = 0;
+    virtual void set_steps(double tau, double sigma) = 0;
+    // Anchors = current point, Halpern counter = 0, first primal step from the current dual (after set_steps).
+    virtual void restart() = 0;
+    virtual void iterate(int steps) = 0;
+    virtual void eval(double out[NKKT]) = 0;
+    // base += inner point (FP64).
+    virtual void end_round() = 0;
+    // base + inner point in FP64, scaled space; either may be null.
+    virtual void download(double* x, double* z) = 0;
+    virtual void set_profile(bool on) = 0;
+    virtual KernelTimes kernel_times() = 0;
+    virtual const std::string& name() const = 0;
+    virtual size_t mem_used() const = 0;
+    virtual int precision_bits() const = 0;
+};
+
+struct FastOptions {
+    int precision_bits = 32;  // iterate type: 32 (FP32 iterates and SpMV, FP64 evaluation and base) or 64
+    int eval_freq = 64;
+    bool use_graph = true;
+    double rho = 1.9;  // reflection coefficient of the Halpern iteration, in [1, 2)
+};
+
+std::unique_ptr<FastDevice> make_fast_device(const Scaled& P, const FastOptions& opt);
+
 // Number of visible HIP devices (0 when none); never throws.
 int device_count();
 

@@ -134,6 +134,133 @@ Outcome run_pdhg(Device& b, const Scaled& P, const Options& o, double time_budge
     return r;
 }
 
+Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_budget, const double* x0, const double* z0,
+                 double omega0, double norm_a) {
+    Outcome r;
+    auto t0 = Clock::now();
+    b.set_profile(o.profile_kernels);
+    r.norm_a = norm_a > 0 ? norm_a : b.power_norm(200);
+    r.power_s = since(t0);
+    const double eta = r.norm_a > 0 ? 0.95 / r.norm_a : 1.0;
+    double cs = 0, bs = 0;
+    for (double v : P.c) cs += v * v;
+    for (int i = 0; i < P.m; ++i) {
+        const double v = row_bound_measure(P.rl[size_t(i)], P.ru[size_t(i)]);
+        bs += v * v;
+    }
+    cs = std::sqrt(cs), bs = std::sqrt(bs);
+    double w = cs > 1e-10 && bs > 1e-10 ? cs / bs : 1.0;
+    if (omega0 > 0) w = omega0;
+    b.set_base(x0, z0);
+    auto wkkt = [&](const double* k) {
+        const double g = k[POBJ] - (k[DROW] + k[DCOL]);
+        return std::sqrt(w * k[PR2S] + k[DR2S] / w + g * g);
+    };
+    auto rel = [&](const double* k) {
+        const double po = k[POBJ], du = k[DROW] + k[DCOL];
+        const double prow = o.row_rel > 0 ? std::sqrt(k[PR2R]) / o.row_rel : 0;
+        return Rel{std::max(std::sqrt(k[PR2O]) / (1 + P.bnorm), prow), std::sqrt(k[DR2O]) / (1 + P.cnorm),
+                   std::fabs(po - du) / (1 + std::fabs(po) + std::fabs(du))};
+    };
+    double kk[NKKT];
+    int64_t it = 0;
+    Rel best{kInf, kInf, kInf};
+    bool done = false;
+    for (int round = 0; !done; ++round) {
+        const bool last_round = !o.refine || round >= o.max_refinements;
+        b.begin_round();
+        b.set_steps(eta / w, eta * w);
+        b.restart();
+        b.eval(kk);
+        best = rel(kk);
+        const double start = best.worst();
+        if (start <= o.tol) {
+            r.status = "near_optimal";
+            break;
+        }
+        const double target = (!last_round && round == 0) ? std::max(o.tol, start * o.inner_tol) : o.tol;
+        double k_last = wkkt(kk), k_prev = kInf, best_round = start;
+        int64_t n_since = 0, round_it = 0, best_at = 0;
+        for (;;) {
+            const int steps = int(std::min<int64_t>(o.eval_freq, o.max_iter - it));
+            if (steps <= 0) {
+                done = true;
+                break;
+            }
+            b.iterate(steps);
+            it += steps, n_since += steps, round_it += steps;
+            b.eval(kk);
+            const Rel rc = rel(kk);
+            best = rc;
+            if (o.verbose && (it / o.eval_freq) % 50 == 0)
+                std::fprintf(stderr, "it %ld  round %d  w %.3e  p %.2e d %.2e g %.2e  pobj %.10g\n", long(it), round, w,
+                             rc.p, rc.d, rc.g, kk[POBJ]);
+            if (!std::isfinite(rc.worst())) {
+                r.status = "numerical_failure";
+                done = true;
+                break;
+            }
+            if (rc.worst() <= o.tol) {
+                // Confirm in FP64: fold the inner point into the base and evaluate with exact FP64 products.
+                b.end_round();
+                b.begin_round();
+                b.set_steps(eta / w, eta * w);
+                b.restart();
+                b.eval(kk);
+                best = rel(kk);
+                if (best.worst() <= o.tol) {
+                    r.status = "near_optimal";
+                    done = true;
+                    break;
+                }
+                r.refinements++;  // not yet: continue from this point as a new round
+                k_last = wkkt(kk), k_prev = kInf, n_since = 0, best_round = best.worst(), best_at = round_it;
+                continue;
+            }
+            if (since(t0) > time_budget) {
+                r.status = "time_limit";
+                done = true;
+                break;
+            }
+            if (it >= o.max_iter) {
+                done = true;
+                break;
+            }
+            if (!last_round) {
+                if (rc.worst() <= target) break;  // recentre
+                if (rc.worst() < 0.5 * best_round) best_round = rc.worst(), best_at = round_it;
+                // stalled (no halving for as many iterations as the last halving took, at least 8192): recentre
+                if (round_it - best_at >= std::max<int64_t>(8192, best_at)) break;
+            }
+            // Restart tests of gpu/pdhg.cu:827-842 with the current point as the candidate.
+            const double kc = wkkt(kk);
+            const bool restart =
+                double(n_since) >= 0.36 * double(round_it) || kc <= 0.2 * k_last || (kc <= 0.8 * k_last && kc > k_prev);
+            k_prev = kc;
+            if (restart) {
+                const double dx = std::sqrt(kk[DX2]), dz = std::sqrt(kk[DZ2]);
+                if (dx > 1e-10 && dz > 1e-10) {
+                    w = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w));
+                    b.set_steps(eta / w, eta * w);
+                }
+                b.restart();
+                k_last = wkkt(kk), k_prev = kInf, n_since = 0, r.restarts++;
+            }
+        }
+        if (!done) {
+            b.end_round();
+            r.refinements++;
+        }
+    }
+    r.iterations = it;
+    r.rel_primal = best.p, r.rel_dual = best.d, r.rel_gap = best.g, r.omega = w, r.from_average = false;
+    r.x.resize(size_t(P.n)), r.z.resize(size_t(P.m));
+    b.download(r.x.data(), r.z.data());
+    r.iterate_s = since(t0) - r.power_s;
+    r.kernels = b.kernel_times();
+    return r;
+}
+
 Session::Session(const LP& lp, const Options& o, Timings tm) : lp_(lp), o_(o), tm_(tm) {
     created_ = Clock::now();
     auto t = Clock::now();
@@ -146,11 +273,17 @@ Session::Session(const LP& lp, const Options& o, Timings tm) : lp_(lp), o_(o), t
         throw std::runtime_error("no HIP device visible: the PDHG solver runs on an AMD GPU only (there is no CPU path)");
     tm_.init = since(t);
     t = Clock::now();
-    DeviceOptions dopt;
-    dopt.precision_bits = o.precision_bits;
-    dopt.eval_freq = o.eval_freq;
-    dopt.use_graph = o.graph;
-    dev_ = make_device(P_, dopt);
+    if (o.halpern) {
+        FastOptions fo;
+        fo.precision_bits = o.precision_bits, fo.eval_freq = o.eval_freq, fo.use_graph = o.graph, fo.rho = o.rho;
+        fast_ = make_fast_device(P_, fo);
+    } else {
+        DeviceOptions dopt;
+        dopt.precision_bits = o.precision_bits;
+        dopt.eval_freq = o.eval_freq;
+        dopt.use_graph = o.graph;
+        dev_ = make_device(P_, dopt);
+    }
     tm_.transfer = since(t);
     setup_s_ = tm_.parse + tm_.scale + tm_.init + tm_.transfer;
 }
@@ -162,8 +295,8 @@ Report Session::solve(const double* x0_orig, const double* y0_orig) {
     auto t_call = Clock::now();
     rep.tm = tm_;
     rep.setup_s = setup_s_;
-    rep.device_name = dev_->name();
-    rep.device_mem_bytes = dev_->mem_used();
+    rep.device_name = fast_ ? fast_->name() : dev_->name();
+    rep.device_mem_bytes = fast_ ? fast_->mem_used() : dev_->mem_used();
     rep.nnz = P.A.nnz();
     std::vector<double> xs0, zs0;
     if (x0_orig) {
@@ -174,8 +307,12 @@ Report Session::solve(const double* x0_orig, const double* y0_orig) {
         zs0.resize(size_t(lp.m));
         for (int i = 0; i < lp.m; ++i) zs0[size_t(i)] = y0_orig[i] / P.Dr[size_t(i)];
     }
-    rep.out = run_pdhg(*dev_, P, o_, o_.time_limit - setup_s_, x0_orig ? xs0.data() : nullptr,
-                       y0_orig ? zs0.data() : nullptr);
+    if (fast_)
+        rep.out = run_fast(*fast_, P, o_, o_.time_limit - setup_s_, x0_orig ? xs0.data() : nullptr,
+                           y0_orig ? zs0.data() : nullptr);
+    else
+        rep.out = run_pdhg(*dev_, P, o_, o_.time_limit - setup_s_, x0_orig ? xs0.data() : nullptr,
+                           y0_orig ? zs0.data() : nullptr);
     rep.solve_s = since(t_call);
     // Unscale and check the point on the ORIGINAL model in FP64 (gpu/pdhg.cu:866-876).
     rep.x.resize(size_t(lp.n)), rep.y.resize(size_t(lp.m));
@@ -226,7 +363,10 @@ std::string report_json(const Report& r, const LP& lp, const Options& o) {
         r.tm.scale, r.tm.init, r.tm.transfer, r.solve_s, q.power_s, r.total_s, r.device_name.c_str(), "gpu", 0,
         o.precision_bits == 32 ? "fp32" : "fp64", lp.m, lp.n, long(r.nnz), q.norm_a, q.omega, o.eval_freq,
         o.graph ? "true" : "false", r.device_mem_bytes,
-        o.precision_bits == 32 ? (o.refine ? "fp32+fp64-refinement" : "fp32") : "fp64", q.refinements, iters_per_s,
+        (std::string(o.halpern ? "halpern-" : "") +
+         (o.precision_bits == 32 ? (o.refine ? "fp32+fp64-refinement" : "fp32") : (o.refine ? "fp64+refinement" : "fp64")))
+            .c_str(),
+        q.refinements, iters_per_s,
         r.solve_s > 0 ? 1.0 / r.solve_s : 0.0);
     return buf;
 }

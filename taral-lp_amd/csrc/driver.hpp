@@ -43,6 +43,8 @@ struct Options {  // defaults of gpu/pdhg.cu:744-750 plus this build's additions
     double inner_tol = 1e-4;   // relative KKT target of each refinement round's inner solve
     int max_refinements = 8;
     bool profile_kernels = false;
+    bool halpern = false;      // fast path: reflected Halpern PDHG on the FP64-recentred problem
+    double rho = 1.9;          // its reflection coefficient
 };
 
 struct Outcome {
@@ -58,6 +60,17 @@ struct Outcome {
 // Restarted averaged PDHG on the device backend (gpu/pdhg.cu:761-849). x0/z0 (scaled space, nullable) warm start
 // the iterates; omega0 > 0 replaces the initial primal weight. norm_a > 0 skips the power iteration.
 Outcome run_pdhg(Device& dev, const Scaled& P, const Options& o, double time_budget, const double* x0 = nullptr,
+                 const double* z0 = nullptr, double omega0 = 0, double norm_a = 0);
+
+// The fast path (Options::halpern): reflected Halpern PDHG iterations on the FastDevice with the same restart
+// tests and primal-weight update as run_pdhg, FP64 evaluation at base + inner point, and, when o.refine, FP64
+// iterative refinement: the first round ends when its relative KKT has dropped by inner_tol, any round ends when
+// it stalls (no halving for as many iterations as its last halving took, at least 8192) — FP32 products put a
+// floor under the first round's evaluated gap; the base then moves to the round's point and the next round
+// solves the recentred problem with exact FP64 base products, keeping the primal weight. A point that meets the
+// tolerance is confirmed by an FP64 evaluation at base + inner before the solve stops. x0/z0 (scaled
+// space, nullable) become the first base point (warm start).
+Outcome run_fast(FastDevice& dev, const Scaled& P, const Options& o, double time_budget, const double* x0 = nullptr,
                  const double* z0 = nullptr, double omega0 = 0, double norm_a = 0);
 
 struct Timings {
@@ -82,7 +95,8 @@ public:
     Session(const LP& lp, const Options& o, Timings tm = Timings());
     Report solve(const double* x0_orig = nullptr, const double* y0_orig = nullptr);
     const Scaled& scaled() const { return P_; }
-    Device& device() { return *dev_; }
+    Device* device() { return dev_.get(); }
+    FastDevice* fast_device() { return fast_.get(); }
     const Options& options() const { return o_; }
 
 private:
@@ -91,6 +105,7 @@ private:
     Timings tm_;
     Scaled P_;
     std::unique_ptr<Device> dev_;
+    std::unique_ptr<FastDevice> fast_;
     double setup_s_ = 0;
     std::chrono::steady_clock::time_point created_;
 };

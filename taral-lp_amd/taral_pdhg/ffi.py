@@ -29,6 +29,9 @@ EXPORTS = [
     "tpdhg_dev_create", "tpdhg_dev_power_norm", "tpdhg_dev_set_steps", "tpdhg_dev_start", "tpdhg_dev_iterate",
     "tpdhg_dev_eval", "tpdhg_dev_eval2", "tpdhg_dev_restart", "tpdhg_dev_download", "tpdhg_dev_spmv",
     "tpdhg_dev_info", "tpdhg_dev_set_profile", "tpdhg_dev_kernel_times", "tpdhg_dev_destroy",
+    "tpdhg_fast_create", "tpdhg_fast_power_norm", "tpdhg_fast_set_base", "tpdhg_fast_begin_round",
+    "tpdhg_fast_set_steps", "tpdhg_fast_restart", "tpdhg_fast_iterate", "tpdhg_fast_eval", "tpdhg_fast_end_round",
+    "tpdhg_fast_download", "tpdhg_fast_destroy",
     "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
     "tpdhg_result_json", "tpdhg_spmv_bench",
 ]
@@ -62,7 +65,8 @@ class Options(C.Structure):
     _fields_ = [("tol", C.c_double), ("time_limit", C.c_double), ("max_iter", C.c_int64),
                 ("eval_freq", C.c_int32), ("row_rel", C.c_double), ("precision_bits", C.c_int32),
                 ("refine", C.c_int32), ("inner_tol", C.c_double), ("max_refinements", C.c_int32),
-                ("use_graph", C.c_int32), ("verbose", C.c_int32), ("profile_kernels", C.c_int32)]
+                ("use_graph", C.c_int32), ("verbose", C.c_int32), ("profile_kernels", C.c_int32),
+                ("halpern", C.c_int32), ("reflection", C.c_double)]
 
 
 class Result(C.Structure):
@@ -120,6 +124,16 @@ def lib():
         L.tpdhg_dev_set_profile.argtypes = [C.c_void_p, C.c_int32]
         L.tpdhg_dev_kernel_times.argtypes = [C.c_void_p, _d, _d, _i64, _i64]
         L.tpdhg_dev_destroy.argtypes = [C.c_void_p]
+        L.tpdhg_fast_create.argtypes = [C.POINTER(ScaledArrays), C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                        C.POINTER(C.c_void_p)]
+        L.tpdhg_fast_power_norm.argtypes = [C.c_void_p, C.c_int32, _d]
+        L.tpdhg_fast_set_base.argtypes = [C.c_void_p, _d, _d]
+        for nm in ("begin_round", "restart", "end_round", "destroy"):
+            getattr(L, "tpdhg_fast_" + nm).argtypes = [C.c_void_p]
+        L.tpdhg_fast_set_steps.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.tpdhg_fast_iterate.argtypes = [C.c_void_p, C.c_int32]
+        L.tpdhg_fast_eval.argtypes = [C.c_void_p, _d]
+        L.tpdhg_fast_download.argtypes = [C.c_void_p, _d, _d]
         L.tpdhg_options_default.argtypes = [C.POINTER(Options)]
         L.tpdhg_solve.argtypes = [C.c_void_p, C.POINTER(Options), _d, _d, C.POINTER(Result), _d, _d]
         L.tpdhg_session_create.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(C.c_void_p)]
@@ -316,6 +330,57 @@ class Device:
         na, nb = C.c_int64(), C.c_int64()
         _check(lib().tpdhg_dev_kernel_times(self._h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb)))
         return dict(row_ms=a.value, col_ms=b.value, row_launches=na.value, col_launches=nb.value)
+
+
+class FastDevice:
+    """The fast-path backend (reflected Halpern PDHG around an FP64 base point) on the current HIP device."""
+
+    def __init__(self, scaled: Scaled, precision_bits=32, eval_freq=64, use_graph=True, reflection=1.9):
+        self.scaled = scaled
+        self.m, self.n = scaled.view.m, scaled.view.n
+        h = C.c_void_p()
+        _check(lib().tpdhg_fast_create(C.byref(scaled.view), int(precision_bits), int(eval_freq),
+                                       int(bool(use_graph)), float(reflection), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.tpdhg_fast_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def power_norm(self, iters=200):
+        r = C.c_double()
+        _check(lib().tpdhg_fast_power_norm(self._h, iters, C.byref(r)))
+        return r.value
+
+    def set_base(self, x_b=None, z_b=None):
+        x_b, z_b = _f64(x_b), _f64(z_b)
+        _check(lib().tpdhg_fast_set_base(self._h, _p(x_b), _p(z_b)))
+
+    def begin_round(self):
+        _check(lib().tpdhg_fast_begin_round(self._h))
+
+    def set_steps(self, tau, sigma):
+        _check(lib().tpdhg_fast_set_steps(self._h, float(tau), float(sigma)))
+
+    def restart(self):
+        _check(lib().tpdhg_fast_restart(self._h))
+
+    def iterate(self, steps):
+        _check(lib().tpdhg_fast_iterate(self._h, int(steps)))
+
+    def eval(self):
+        out = np.zeros(10)
+        _check(lib().tpdhg_fast_eval(self._h, _p(out)))
+        return out
+
+    def end_round(self):
+        _check(lib().tpdhg_fast_end_round(self._h))
+
+    def download(self):
+        x, z = np.zeros(max(self.n, 1)), np.zeros(max(self.m, 1))
+        _check(lib().tpdhg_fast_download(self._h, _p(x), _p(z)))
+        return x[:self.n], z[:self.m]
 
 
 def default_options(**kw) -> Options:

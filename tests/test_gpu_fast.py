@@ -1,0 +1,113 @@
+"""The fast path (reflected Halpern PDHG on an FP64-recentred problem, FP32 iterates, FP64 refinement) on the GPU.
+
+No reference counterpart exists for these iterations, so they are checked two ways: one iteration block of the
+FP64 kernels against the iteration written out in numpy from its definition (include/taral_pdhg.h, fast-path
+section), with and without a base point and on inequality rows; and full FP32+refinement solves against the
+planted optimum, the reference dual simplex's optimum, the reference ledgers' exact optima, and an independent
+original-space KKT recomputation of the returned point (tests/kkt_check.py) at the requested 1e-6.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from kkt_check import rel_kkt
+from oracle import oracle
+from taral_pdhg import ffi, gen
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _halpern_numpy(S, tau, sig, rho, steps, xb, zb):
+    """`steps` reflected Halpern iterations in FP64 on the problem recentred at (xb, zb), from dx = clamp(0),
+    dz = 0 (anchors there). Returns the combined point."""
+    ptr, idx, val = S.csr("A")
+    m, n = S.view.m, S.view.n
+    A = sp.csr_matrix((val, idx, ptr), shape=(m, n))
+    c, l, u, rl, ru = (S.vec(k) for k in ("c", "l", "u", "rl", "ru"))
+    c_in, l_in, u_in = c + A.T @ zb, l - xb, u - xb
+    rl_in, ru_in = rl - A @ xb, ru - A @ xb
+    x = np.clip(np.zeros(n), l_in, u_in)
+    z = np.zeros(m)
+    x0, z0 = x.copy(), z.copy()
+    for k in range(steps):
+        lam = (k + 1) / (k + 2)
+        xp = np.clip(x - tau * (c_in + A.T @ z), l_in, u_in)
+        v = z + sig * (A @ (2 * xp - x))
+        q = (v + zb) / sig
+        zp = np.where(q < rl_in, v - sig * rl_in, np.where(q > ru_in, v - sig * ru_in, -zb))
+        x = lam * (rho * xp - (rho - 1) * x) + (1 - lam) * x0
+        z = lam * (rho * zp - (rho - 1) * z) + (1 - lam) * z0
+    return xb + x, zb + z, xb + x0, zb + z0
+
+
+@pytest.mark.parametrize("which", ["cfg0", "packing"])
+@pytest.mark.parametrize("with_base", [False, True])
+def test_halpern_block_fp64(which, with_base, cfg0):
+    md = cfg0 if which == "cfg0" else gen.packing_mps(10000)[0]
+    print(f"halpern block {which} base={with_base}", flush=True)
+    S = ffi.Scaled(ffi.Model.from_lp(md))
+    O = oracle.OracleLP(md)
+    fd = ffi.FastDevice(S, precision_bits=64, eval_freq=16, use_graph=True, reflection=1.9)
+    norm = fd.power_norm()
+    tau, sig = 0.95 / norm / 0.7, 0.95 / norm * 0.7
+    rng = np.random.default_rng(3)
+    xb = np.clip(rng.random(S.view.n), S.vec("l"), S.vec("u")) if with_base else np.zeros(S.view.n)
+    zb = rng.standard_normal(S.view.m) * 0.1 if with_base else np.zeros(S.view.m)
+    fd.set_base(xb, zb)
+    fd.begin_round()
+    fd.set_steps(tau, sig)
+    fd.restart()
+    fd.iterate(16)  # graph-captured block
+    x, z = fd.download()
+    xw, zw, xa, za = _halpern_numpy(S, tau, sig, 1.9, 16, xb, zb)
+    scale = max(1.0, np.abs(xw).max(), np.abs(zw).max())
+    assert np.max(np.abs(x - xw)) <= 1e-10 * scale and np.max(np.abs(z - zw)) <= 1e-10 * scale
+    # FP64 evaluation of the combined point equals the reference evaluation of the same vectors
+    ptr, idx, val = S.csr("A")
+    A = sp.csr_matrix((val, idx, ptr), shape=(S.view.m, S.view.n))
+    want = O.eval(A @ xw, zw, xw, A.T @ zw, xa, za)
+    got = fd.eval()
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(want).max()))
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-6])
+def test_fast_solve_planted(cfg0, tol):
+    print(f"fast cfg0 tol={tol}", flush=True)
+    r = ffi.solve(ffi.Model.from_lp(cfg0), tol=tol, precision_bits=32, halpern=1, refine=1)
+    assert r["status"] == "near_optimal"
+    p, d, g = rel_kkt(cfg0, r["x"], r["y"])
+    assert max(p, d, g) <= tol * 1.0001
+    assert r["rel_primal"] == pytest.approx(p, rel=1e-6, abs=1e-12) and r["rel_gap"] == pytest.approx(g, rel=1e-6, abs=1e-12)
+    ds = json.load(open(os.path.join(GOLDEN, "cfg0_dual_simplex.json")))["objective"]
+    for ref in (cfg0.planted_objective, ds):
+        assert abs(r["objective"] - ref) / (1 + abs(ref)) <= 3 * tol
+    assert np.all(r["x"] >= -1e-9) and np.all(r["x"] <= 10 + 1e-9)
+
+
+@pytest.mark.parametrize("case", ["packing_10000", "transport_100x100"])
+def test_fast_solve_inequality_rows(case):
+    """L and G rows: the recentred row prox must keep the dual sign conditions exact."""
+    led = json.load(open(os.path.join(GOLDEN, "ledger_pdhg.json")))
+    exact = next(c for c in led["cases"] if c["case"] == case)["exact_objective"]
+    md = gen.packing_mps(10000)[0] if case.startswith("packing") else gen.transport_mps(100, 100)[0]
+    print(f"fast {case}", flush=True)
+    r = ffi.solve(ffi.Model.from_lp(md), tol=1e-6, precision_bits=32, halpern=1, refine=1)
+    assert r["status"] == "near_optimal"
+    assert max(rel_kkt(md, r["x"], r["y"])) <= 1.0001e-6
+    assert abs(r["objective"] - exact) / (1 + abs(exact)) <= 3e-6
+
+
+def test_fast_warm_start_at_optimum(cfg0):
+    r = ffi.solve(ffi.Model.from_lp(cfg0), x0=cfg0.x_star, y0=-cfg0.y_star, tol=1e-6, precision_bits=32, halpern=1,
+                  refine=1)
+    assert r["status"] == "near_optimal" and r["iterations"] == 0
+
+
+def test_fast_fp64_without_refinement(cfg0):
+    r = ffi.solve(ffi.Model.from_lp(cfg0), tol=1e-6, precision_bits=64, halpern=1)
+    assert r["status"] == "near_optimal" and r["refinements"] == 0
+    assert max(rel_kkt(cfg0, r["x"], r["y"])) <= 1.0001e-6
