@@ -229,8 +229,11 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
             if (!last_round) {
                 if (rc.worst() <= target) break;  // recentre
                 if (rc.worst() < 0.5 * best_round) best_round = rc.worst(), best_at = round_it;
-                // stalled (no halving for as many iterations as the last halving took, at least 8192): recentre
-                if (round_it - best_at >= std::max<int64_t>(8192, best_at)) break;
+                // Stalled: no halving for as many iterations as the last halving took (at least 8192) in the first
+                // round, whose FP32 products floor the evaluated gap; later rounds see FP64 base products and only
+                // recentre after twice that (at least 32768).
+                const int64_t wait = round == 0 ? std::max<int64_t>(8192, best_at) : std::max<int64_t>(32768, 2 * best_at);
+                if (round_it - best_at >= wait) break;
             }
             // Restart tests of gpu/pdhg.cu:827-842 with the current point as the candidate.
             const double kc = wkkt(kk);

@@ -64,9 +64,9 @@ Outcome run_pdhg(Device& dev, const Scaled& P, const Options& o, double time_bud
 
 // The fast path (Options::halpern): reflected Halpern PDHG iterations on the FastDevice with the same restart
 // tests and primal-weight update as run_pdhg, FP64 evaluation at base + inner point, and, when o.refine, FP64
-// iterative refinement: the first round ends when its relative KKT has dropped by inner_tol, any round ends when
-// it stalls (no halving for as many iterations as its last halving took, at least 8192) — FP32 products put a
-// floor under the first round's evaluated gap; the base then moves to the round's point and the next round
+// iterative refinement: the first round ends when its relative KKT has dropped by inner_tol or when it stalls (no
+// halving for as many iterations as its last halving took, at least 8192) — FP32 products put a floor under the
+// first round's evaluated gap; later rounds end only on a longer stall (twice that, at least 32768); the base moves to the round's point and the next round
 // solves the recentred problem with exact FP64 base products, keeping the primal weight. A point that meets the
 // tolerance is confirmed by an FP64 evaluation at base + inner before the solve stops. x0/z0 (scaled
 // space, nullable) become the first base point (warm start).
