@@ -155,8 +155,8 @@ public:
         HIP_OK(hipMemGetInfo(&free0, &total));
         name_ = device_name();
         HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-        A_ = upload_csr<T>(mem_, st_, m, a_ptr, a_idx, a_val);
-        AT_ = upload_csr<T>(mem_, st_, n, t_ptr, t_idx, t_val);
+        A_ = upload_csr<T>(mem_, st_, m, n, a_ptr, a_idx, a_val);
+        AT_ = upload_csr<T>(mem_, st_, n, m, t_ptr, t_idx, t_val);
         hl_.assign(l_, l_ + n), hu_.assign(u_, u_ + n);
         c = up(c_, n), l = up(l_, n), u = up(u_, n), rl = up(rl_, m), ru = up(ru_, m), Dr = up(Dr_, m), Dc = up(Dc_, n);
         par = zeros(2);
@@ -456,7 +456,7 @@ void spmv_bench_t(int rows, int cols, const int32_t* ptr, const int32_t* idx, co
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } guard{st};
-    DevMat<T> a = upload_csr<T>(mem, st, rows, ptr, idx, val);
+    DevMat<T> a = upload_csr<T>(mem, st, rows, cols, ptr, idx, val);
     T* dv = mem.alloc<T>(size_t(cols));
     T* dy = mem.alloc<T>(size_t(rows));
     double* dy64 = mem.alloc<double>(size_t(rows));

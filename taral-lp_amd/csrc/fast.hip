@@ -166,10 +166,14 @@ public:
         HIP_OK(hipMemGetInfo(&free0, &total));
         name_ = device_name();
         HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-        A_ = upload_csr<T>(mem_, st_, m_, P.A.ptr.data(), P.A.idx.data(), P.A.val.data());
-        AT_ = upload_csr<T>(mem_, st_, n_, P.AT.ptr.data(), P.AT.idx.data(), P.AT.val.data());
-        A64_ = values64(A_, P.A);
-        AT64_ = values64(AT_, P.AT);
+        {
+            const HostLayout LA = make_layout(m_, n_, P.A.ptr.data(), P.A.idx.data());
+            const HostLayout LT = make_layout(n_, m_, P.AT.ptr.data(), P.AT.idx.data());
+            A_ = upload_layout<T>(mem_, st_, m_, LA, P.A.val.data());
+            AT_ = upload_layout<T>(mem_, st_, n_, LT, P.AT.val.data());
+            A64_ = values64(A_, LA, P.A);
+            AT64_ = values64(AT_, LT, P.AT);
+        }
         c64 = up64(P.c), l64 = up64(P.l), u64 = up64(P.u), rl64 = up64(P.rl), ru64 = up64(P.ru);
         Dr64 = up64(P.Dr), Dc64 = up64(P.Dc);
         xb = zeros64(n_), zb = zeros64(m_), Axb = zeros64(m_), ATzb = zeros64(n_), buf64 = zeros64(std::max(m_, n_));
@@ -336,19 +340,17 @@ private:
         HIP_OK(hipStreamSynchronize(st_));
         return p;
     }
-    // FP64 copy of a matrix's values sharing its pointers and blocks (the same buffer when T is double).
-    DevMat<double> values64(const DevMat<T>& a, const Csr& h) {
-        DevMat<double> d;
-        d.rows = a.rows, d.nblk = a.nblk, d.ptr = a.ptr, d.idx = a.idx, d.blk = a.blk;
+    // FP64 copy of a matrix's values sharing its pointers and blocks (the same buffers when T is double).
+    DevMat<double> values64(const DevMat<T>& a, const HostLayout& L, const Csr& h) {
         if constexpr (sizeof(T) == sizeof(double)) {
-            d.val = reinterpret_cast<double*>(a.val);
+            DevMat<double> d;
+            d.rows = a.rows, d.nblk = a.nblk, d.maxblk = a.maxblk, d.slab = a.slab;
+            d.ptr = a.ptr, d.idx = a.idx, d.blk = a.blk, d.snblk = a.snblk, d.sboff = a.sboff;
+            d.val = reinterpret_cast<double*>(a.val), d.part = reinterpret_cast<double*>(a.part);
+            return d;
         } else {
-            d.val = mem_.alloc<double>(std::max<size_t>(h.val.size(), 1));
-            if (!h.val.empty())
-                HIP_OK(hipMemcpyAsync(d.val, h.val.data(), h.val.size() * sizeof(double), hipMemcpyHostToDevice, st_));
-            HIP_OK(hipStreamSynchronize(st_));
+            return share_structure<double>(mem_, st_, a, L, h.val.data());
         }
-        return d;
     }
     void h2d(T* dst, const T* src, size_t k) {
         if (!k) return;

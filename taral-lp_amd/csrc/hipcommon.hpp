@@ -18,6 +18,7 @@ namespace {
 constexpr int kWG = 256;    // threads per workgroup: 4 waves of 64
 constexpr int kCap = 2048;  // nonzeros staged per row block (8 per thread)
 constexpr int kMaxRedBlocks = 1024;
+constexpr int kSlabsK = 8;  // column slabs of the slab layout: one per XCD
 
 #define HIP_OK(call)                                                                                      \
     do {                                                                                                  \
@@ -81,6 +82,64 @@ __global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restri
     }
     for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (row < nr && lane == 0) epi(r0 + row, s);
+}
+
+// Slab layout, first kernel: workgroup b handles row block b / 8 of slab b % 8 (slab = XCD) and writes the
+// slab's partial row sums part[slab * rows + row]; same staging and reduction as k_rowblocks.
+template <class T>
+__global__ __launch_bounds__(kWG) void k_slab_partial(int rows, const int* __restrict__ snblk,
+                                                      const int* __restrict__ sboff, const int* __restrict__ blk_all,
+                                                      const int* __restrict__ ptr_all, const int* __restrict__ idx,
+                                                      const T* __restrict__ val, const T* __restrict__ v,
+                                                      T* __restrict__ part) {
+    __shared__ T sh[kCap];
+    __shared__ T red[kWG / 64];
+    const int q = int(blockIdx.x) & (kSlabsK - 1), lb = int(blockIdx.x) / kSlabsK;
+    if (lb >= snblk[q]) return;
+    const int* blk = blk_all + sboff[q];
+    const int* ptr = ptr_all + size_t(q) * (size_t(rows) + 1);
+    T* y = part + size_t(q) * size_t(rows);
+    const int tid = int(threadIdx.x);
+    const int r0 = blk[lb], r1 = blk[lb + 1];
+    const int k0 = ptr[r0], k1 = ptr[r1];
+    if (k1 - k0 > kCap) {
+        T s = 0;
+        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * v[idx[k]];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) y[r0] = (red[0] + red[1]) + (red[2] + red[3]);
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < kCap / kWG; ++u) {
+        const int k = k0 + u * kWG + tid;
+        if (k < k1) sh[k - k0] = val[k] * v[idx[k]];
+    }
+    __syncthreads();
+    const int nr = r1 - r0;
+    int lg = 6;
+    while (lg > 0 && (nr << lg) > kWG) --lg;
+    const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
+    T s = 0;
+    if (row < nr) {
+        const int a = ptr[r0 + row] - k0, e = ptr[r0 + row + 1] - k0;
+        for (int k = a + lane; k < e; k += lpr) s += sh[k];
+    }
+    for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (row < nr && lane == 0) y[r0 + row] = s;
+}
+
+// Slab layout, second kernel: the eight partial sums of each row in slab order, then the epilogue.
+template <class T, class Epi>
+__global__ __launch_bounds__(kWG) void k_combine_epi(int rows, const T* __restrict__ part, Epi epi) {
+    epi.load();
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < rows; i += int(gridDim.x * blockDim.x)) {
+        T s = part[i];
+#pragma unroll
+        for (int q = 1; q < kSlabsK; ++q) s += part[size_t(q) * size_t(rows) + size_t(i)];
+        epi(i, s);
+    }
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -195,36 +254,138 @@ struct Mem {  // device allocations owned by one backend
     }
 };
 
-template <class T>
-struct DevMat {
-    int rows = 0, nblk = 0;
-    int *ptr = nullptr, *idx = nullptr, *blk = nullptr;
-    T* val = nullptr;
-};
+// One CSR on the device in one of two layouts.
+//  plain: row blocks over the whole matrix (k_rowblocks).
+//  slab:  for a gathered vector of at least kSlabMinCols entries (half an XCD's L2 in FP32), the columns are split into kSlabs equal ranges
+//         and the matrix stored slab-major as kSlabs CSR pieces with absolute row pointers. Workgroup b works on
+//         slab b % 8, i.e. on the XCD the hardware sends it to, so each XCD gathers from a 1/8 slice of the
+//         vector that stays in its own L2; each slab writes partial row sums and a combine kernel adds the eight
+//         in a fixed order and applies the epilogue.
+constexpr int kSlabs = kSlabsK;
+constexpr int kSlabMinCols = 1 << 19;  // 2 MB of FP32 gather target and up
 
 template <class T>
-DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, const int32_t* ptr, const int32_t* idx, const double* val) {
-    DevMat<T> d;
-    d.rows = rows;
+struct DevMat {
+    int rows = 0, nblk = 0, maxblk = 0;
+    bool slab = false;
+    int *ptr = nullptr, *idx = nullptr, *blk = nullptr;  // slab: ptr holds kSlabs * (rows + 1) entries
+    int *snblk = nullptr, *sboff = nullptr;              // slab: block count and block-list offset per slab
+    T* val = nullptr;
+    T* part = nullptr;  // slab: kSlabs * rows partial sums
+};
+
+struct HostLayout {
+    bool slab = false;
+    int nblk = 0, maxblk = 0;
+    std::vector<int32_t> ptr, idx, blk, snblk, sboff;
+    std::vector<int64_t> perm;  // slab layout: position k holds the CSR entry perm[k]
+};
+
+HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* idx) {
+    HostLayout L;
     const int64_t nnz = ptr[rows];
-    std::vector<int32_t> blk = row_blocks(rows, ptr);
-    d.nblk = int(blk.size()) - 1;
-    d.ptr = mem.alloc<int>(size_t(rows) + 1), d.idx = mem.alloc<int>(size_t(nnz)), d.blk = mem.alloc<int>(blk.size());
-    d.val = mem.alloc<T>(size_t(nnz));
-    HIP_OK(hipMemcpyAsync(d.ptr, ptr, (size_t(rows) + 1) * sizeof(int), hipMemcpyHostToDevice, st));
-    if (nnz) HIP_OK(hipMemcpyAsync(d.idx, idx, size_t(nnz) * sizeof(int), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(d.blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    L.slab = cols >= kSlabMinCols && rows > 0;
+    if (!L.slab) {
+        L.ptr.assign(ptr, ptr + rows + 1);
+        L.idx.assign(idx, idx + nnz);
+        L.blk = row_blocks(rows, ptr);
+        L.nblk = int(L.blk.size()) - 1;
+        return L;
+    }
+    auto slab_of = [&](int32_t c) { return int(int64_t(c) * kSlabs / cols); };
+    std::vector<std::vector<int32_t>> sp(kSlabs, std::vector<int32_t>(size_t(rows) + 1, 0));
+    for (int i = 0; i < rows; ++i)
+        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) sp[size_t(slab_of(idx[k]))][size_t(i) + 1]++;
+    std::vector<int64_t> soff(kSlabs + 1, 0);
+    for (int q = 0; q < kSlabs; ++q) {
+        for (int i = 0; i < rows; ++i) sp[size_t(q)][size_t(i) + 1] += sp[size_t(q)][size_t(i)];
+        soff[size_t(q) + 1] = soff[size_t(q)] + sp[size_t(q)][size_t(rows)];
+    }
+    L.idx.resize(size_t(nnz)), L.perm.resize(size_t(nnz));
+    std::vector<int64_t> next(kSlabs * size_t(rows));
+    for (int q = 0; q < kSlabs; ++q)
+        for (int i = 0; i < rows; ++i) next[size_t(q) * size_t(rows) + size_t(i)] = soff[size_t(q)] + sp[size_t(q)][size_t(i)];
+    for (int i = 0; i < rows; ++i)
+        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+            const int q = slab_of(idx[k]);
+            const int64_t at = next[size_t(q) * size_t(rows) + size_t(i)]++;
+            L.idx[size_t(at)] = idx[k], L.perm[size_t(at)] = k;
+        }
+    L.ptr.resize(size_t(kSlabs) * (size_t(rows) + 1));
+    L.snblk.resize(kSlabs), L.sboff.resize(kSlabs);
+    for (int q = 0; q < kSlabs; ++q) {
+        for (int i = 0; i <= rows; ++i)
+            L.ptr[size_t(q) * (size_t(rows) + 1) + size_t(i)] = int32_t(soff[size_t(q)] + sp[size_t(q)][size_t(i)]);
+        std::vector<int32_t> b = row_blocks(rows, sp[size_t(q)].data());
+        L.snblk[size_t(q)] = int(b.size()) - 1;
+        L.sboff[size_t(q)] = int(L.blk.size());
+        L.maxblk = std::max(L.maxblk, L.snblk[size_t(q)]);
+        L.blk.insert(L.blk.end(), b.begin(), b.end());
+    }
+    return L;
+}
+
+template <class T>
+T* upload_values(Mem& mem, hipStream_t st, const HostLayout& L, const double* val, int64_t nnz) {
+    T* d = mem.alloc<T>(size_t(std::max<int64_t>(nnz, 1)));
     if (nnz) {
-        std::vector<T> h(val, val + nnz);
-        HIP_OK(hipMemcpyAsync(d.val, h.data(), size_t(nnz) * sizeof(T), hipMemcpyHostToDevice, st));
+        std::vector<T> h(static_cast<size_t>(nnz));
+        if (L.slab)
+            for (int64_t k = 0; k < nnz; ++k) h[size_t(k)] = T(val[L.perm[size_t(k)]]);
+        else
+            for (int64_t k = 0; k < nnz; ++k) h[size_t(k)] = T(val[k]);
+        HIP_OK(hipMemcpyAsync(d, h.data(), size_t(nnz) * sizeof(T), hipMemcpyHostToDevice, st));
         HIP_OK(hipStreamSynchronize(st));
     }
+    return d;
+}
+
+template <class T>
+DevMat<T> upload_layout(Mem& mem, hipStream_t st, int rows, const HostLayout& L, const double* val) {
+    DevMat<T> d;
+    d.rows = rows, d.slab = L.slab, d.nblk = L.nblk, d.maxblk = L.maxblk;
+    auto up = [&](const std::vector<int32_t>& v) {
+        int* p = mem.alloc<int>(std::max<size_t>(v.size(), 1));
+        if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        return p;
+    };
+    d.ptr = up(L.ptr), d.idx = up(L.idx), d.blk = up(L.blk);
+    if (L.slab) {
+        d.snblk = up(L.snblk), d.sboff = up(L.sboff);
+        d.part = mem.alloc<T>(size_t(kSlabs) * size_t(std::max(rows, 1)));
+    }
+    d.val = upload_values<T>(mem, st, L, val, int64_t(L.idx.size()));
     HIP_OK(hipStreamSynchronize(st));
     return d;
 }
 
+// The same structure with values of another type (U), sharing pointers, indices and blocks.
+template <class U, class T>
+DevMat<U> share_structure(Mem& mem, hipStream_t st, const DevMat<T>& a, const HostLayout& L, const double* val) {
+    DevMat<U> d;
+    d.rows = a.rows, d.nblk = a.nblk, d.maxblk = a.maxblk, d.slab = a.slab;
+    d.ptr = a.ptr, d.idx = a.idx, d.blk = a.blk, d.snblk = a.snblk, d.sboff = a.sboff;
+    d.val = upload_values<U>(mem, st, L, val, int64_t(L.idx.size()));
+    if (a.slab) d.part = mem.alloc<U>(size_t(kSlabs) * size_t(std::max(a.rows, 1)));
+    return d;
+}
+
+template <class T>
+DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, int cols, const int32_t* ptr, const int32_t* idx,
+                     const double* val) {
+    return upload_layout<T>(mem, st, rows, make_layout(rows, cols, ptr, idx), val);
+}
+
 template <class T, class Epi>
 void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
+    if (a.rows == 0) return;
+    if (a.slab) {
+        if (a.maxblk > 0)
+            k_slab_partial<T><<<kSlabs * a.maxblk, kWG, 0, st>>>(a.rows, a.snblk, a.sboff, a.blk, a.ptr, a.idx, a.val,
+                                                                v, a.part);
+        k_combine_epi<T, Epi><<<grid_for(a.rows), kWG, 0, st>>>(a.rows, a.part, epi);
+        return;
+    }
     if (a.nblk == 0) return;
     k_rowblocks<T, Epi><<<a.nblk, kWG, 0, st>>>(a.nblk, a.blk, a.ptr, a.idx, a.val, v, epi);
 }

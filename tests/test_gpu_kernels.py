@@ -32,8 +32,10 @@ def _spmv_ref(ptr, idx, val, v):
     return y, mag
 
 
-SPMV_CASES = {
+SPMV_CASES = {  # row lengths; gathered vectors of 2^19 entries or more take the XCD column-slab layout
     "poisson_rows": lambda rng: rng.poisson(25, 3000),
+    "slab_poisson_rows": lambda rng: rng.poisson(40, 20000),
+    "slab_long_rows": lambda rng: np.concatenate([rng.integers(0, 30, 3000), [30000], rng.integers(0, 30, 300)]),
     "empty_rows": lambda rng: np.where(rng.random(5000) < 0.7, 0, rng.integers(1, 9, 5000)),
     "long_rows": lambda rng: np.concatenate([rng.integers(0, 20, 500), [10240], rng.integers(0, 20, 500), [4097],
                                              [2048], [2049]]),
@@ -48,7 +50,7 @@ SPMV_CASES = {
 def test_spmv_parity(case, bits):
     rng = np.random.default_rng(11)
     lengths = SPMV_CASES[case](rng)
-    cols = 12000
+    cols = 600_000 if case.startswith("slab") else 12000
     ptr, idx, val = _csr_from_rows(lengths, cols, rng)
     v = rng.standard_normal(cols)
     print(f"spmv case={case} bits={bits} rows={ptr.size - 1} nnz={ptr[-1]}", flush=True)
