@@ -197,11 +197,36 @@ typedef struct tpdhg_session tpdhg_session;
 int tpdhg_session_create(const tpdhg_model* model, const tpdhg_options* opt, tpdhg_session** out);
 int tpdhg_session_solve(tpdhg_session* s, const double* x0, const double* y0, tpdhg_result* result, double* x,
                         double* y);
+/* Replaces the cost and the row/column bounds (original space; n, n, n, m, m doubles) of a session's problem
+ * while A stays resident: the vectors are rescaled with the session's Ruiz/Pock-Chambolle factors (which depend
+ * on A only) and uploaded. For scenario families that share A (no reference counterpart). */
+int tpdhg_session_set_vectors(tpdhg_session* s, const double* cost, const double* col_lo, const double* col_up,
+                              const double* row_lo, const double* row_up);
 void tpdhg_session_free(tpdhg_session* s);
 
 /* The JSON object gpu/pdhg.cu:881-897 writes (same keys and formats, plus this build's keys). */
 int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const tpdhg_result* result, char* buf,
                       int32_t len);
+
+/* ------------------------------------------------------------------ learned warm start (no reference counterpart) */
+/* MLP n_in -> h1 -> h2 -> n_out with ReLU, bf16 MFMA matrix products with FP32 accumulation, FP32 master
+ * weights, Adam training on the GPU. n_in, h1, h2: positive multiples of 16. Parameters (FP32, row-major):
+ * W1 [n_in x h1], b1 [h1], W2 [h1 x h2], b2 [h2], W3 [h2 x n_out], b3 [n_out]. bf16 rounding is applied to the
+ * inputs, the weight copies, the hidden activations and the back-propagated errors before each product. */
+typedef struct tpdhg_mlp tpdhg_mlp;
+int tpdhg_mlp_create(int32_t n_in, int32_t h1, int32_t h2, int32_t n_out, uint64_t seed, tpdhg_mlp** out);
+int64_t tpdhg_mlp_param_count(const tpdhg_mlp* net);
+int tpdhg_mlp_set_params(tpdhg_mlp* net, const float* params);
+int tpdhg_mlp_get_params(tpdhg_mlp* net, float* params);
+/* y [batch x n_out] for x [batch x n_in]. */
+int tpdhg_mlp_forward(tpdhg_mlp* net, const float* x, int32_t batch, float* y);
+/* Mean squared error against t [batch x n_out] and its gradient (param_count floats, may be NULL). */
+int tpdhg_mlp_loss_grad(tpdhg_mlp* net, const float* x, const float* t, int32_t batch, double* loss, float* grad);
+/* Adam over shuffled mini-batches of the n samples (x [n x n_in], t [n x n_out]) for `epochs`; the data stays on
+ * the device; *loss = mean loss of the last epoch. */
+int tpdhg_mlp_train(tpdhg_mlp* net, const float* x, const float* t, int32_t n, int32_t batch, int32_t epochs, float lr,
+                    uint64_t seed, double* loss);
+void tpdhg_mlp_destroy(tpdhg_mlp* net);
 
 /* ------------------------------------------------------------------ SpMV microbenchmark (gpu/csr_matvec.cu) */
 /* y = M v for a CSR (rows x cols) on the device with the solver's SpMV kernel; `reps` timed launches after a

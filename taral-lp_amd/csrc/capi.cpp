@@ -9,6 +9,7 @@
 
 #include "../../include/taral_pdhg.h"
 #include "driver.hpp"
+#include "mlp.hpp"
 
 struct tpdhg_model {
     tpdhg::LP lp;
@@ -21,6 +22,9 @@ struct tpdhg_dev {
 };
 struct tpdhg_fast {
     std::unique_ptr<tpdhg::FastDevice> d;
+};
+struct tpdhg_mlp {
+    std::unique_ptr<tpdhg::Mlp> net;
 };
 struct tpdhg_session {
     tpdhg::LP lp;  // the session's own copy; Session keeps a reference to it
@@ -377,6 +381,15 @@ int tpdhg_session_solve(tpdhg_session* s, const double* x0, const double* y0, tp
     });
 }
 
+int tpdhg_session_set_vectors(tpdhg_session* s, const double* cost, const double* col_lo, const double* col_up,
+                              const double* row_lo, const double* row_up) {
+    if (!s || !s->s || !cost || !col_lo || !col_up || !row_lo || !row_up) return fail(TPDHG_ERR_ARG, "null argument");
+    return guarded([&]() -> int {
+        s->s->set_vectors(s->lp, cost, col_lo, col_up, row_lo, row_up);
+        return TPDHG_OK;
+    });
+}
+
 void tpdhg_session_free(tpdhg_session* s) { delete s; }
 
 int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const tpdhg_result* res, char* buf,
@@ -400,6 +413,35 @@ int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const 
         return TPDHG_OK;
     });
 }
+
+int tpdhg_mlp_create(int32_t n_in, int32_t h1, int32_t h2, int32_t n_out, uint64_t seed, tpdhg_mlp** out) {
+    if (!out) return fail(TPDHG_ERR_ARG, "null argument");
+    return guarded([&]() -> int {
+        auto m = std::make_unique<tpdhg_mlp>();
+        m->net = tpdhg::make_mlp(n_in, h1, h2, n_out, seed);
+        *out = m.release();
+        return TPDHG_OK;
+    });
+}
+int64_t tpdhg_mlp_param_count(const tpdhg_mlp* net) { return net && net->net ? net->net->param_count() : -1; }
+#define MLP_CALL(net, body)                                                  \
+    do {                                                                     \
+        if (!(net) || !(net)->net) return fail(TPDHG_ERR_ARG, "null network"); \
+        return guarded([&]() -> int { body; return TPDHG_OK; });             \
+    } while (0)
+int tpdhg_mlp_set_params(tpdhg_mlp* net, const float* p) { MLP_CALL(net, net->net->set_params(p)); }
+int tpdhg_mlp_get_params(tpdhg_mlp* net, float* p) { MLP_CALL(net, net->net->get_params(p)); }
+int tpdhg_mlp_forward(tpdhg_mlp* net, const float* x, int32_t batch, float* y) {
+    MLP_CALL(net, net->net->forward(x, batch, y));
+}
+int tpdhg_mlp_loss_grad(tpdhg_mlp* net, const float* x, const float* t, int32_t batch, double* loss, float* grad) {
+    MLP_CALL(net, { const double l = net->net->loss_grad(x, t, batch, grad); if (loss) *loss = l; });
+}
+int tpdhg_mlp_train(tpdhg_mlp* net, const float* x, const float* t, int32_t n, int32_t batch, int32_t epochs, float lr,
+                    uint64_t seed, double* loss) {
+    MLP_CALL(net, { const double l = net->net->train(x, t, n, batch, epochs, lr, seed); if (loss) *loss = l; });
+}
+void tpdhg_mlp_destroy(tpdhg_mlp* net) { delete net; }
 
 int tpdhg_spmv_bench(int32_t precision_bits, int32_t rows, int32_t cols, const int32_t* ptr, const int32_t* idx,
                      const double* val, const double* v, double* y, int32_t reps, double* mean_ms, double* median_ms) {

@@ -313,6 +313,15 @@ public:
         times_ = KernelTimes();
         return t;
     }
+    void set_vectors(const double* c_, const double* l_, const double* u_, const double* rl_, const double* ru_) override {
+        auto put = [&](T* dst, const double* src, int k) {
+            if (!k) return;
+            std::vector<T> h(src, src + k);
+            h2d(dst, h.data(), size_t(k));
+        };
+        put(c, c_, n_), put(l, l_, n_), put(u, u_, n_), put(rl, rl_, m_), put(ru, ru_, m_);
+        hl_.assign(l_, l_ + n_), hu_.assign(u_, u_ + n_);
+    }
     void synchronize() override { HIP_OK(hipStreamSynchronize(st_)); }
     const std::string& name() const override { return name_; }
     size_t mem_used() const override { return mem_used_; }

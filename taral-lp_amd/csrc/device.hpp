@@ -42,6 +42,8 @@ public:
     virtual void spmv(int which, const double* v, double* y) = 0;
     virtual void set_profile(bool on) = 0;
     virtual KernelTimes kernel_times() = 0;
+    // Replaces c, l, u (n) and rl, ru (m), scaled space; A, A^T and the scaling stay (scenario families).
+    virtual void set_vectors(const double* c, const double* l, const double* u, const double* rl, const double* ru) = 0;
     virtual void synchronize() = 0;
     virtual const std::string& name() const = 0;
     virtual size_t mem_used() const = 0;
@@ -95,6 +97,8 @@ public:
     virtual void download(double* x, double* z) = 0;
     virtual void set_profile(bool on) = 0;
     virtual KernelTimes kernel_times() = 0;
+    // Replaces c, l, u (n) and rl, ru (m), scaled space, FP64; A, A^T and the scaling stay.
+    virtual void set_vectors(const double* c, const double* l, const double* u, const double* rl, const double* ru) = 0;
     virtual const std::string& name() const = 0;
     virtual size_t mem_used() const = 0;
     virtual int precision_bits() const = 0;

@@ -338,6 +338,17 @@ Report Session::solve(const double* x0_orig, const double* y0_orig) {
     return rep;
 }
 
+void Session::set_vectors(LP& lp, const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                          const double* row_up) {
+    if (&lp != &lp_) throw std::runtime_error("set_vectors: not this session's model");
+    lp.cost.assign(cost, cost + lp.n), lp.col_lo.assign(col_lo, col_lo + lp.n), lp.col_up.assign(col_up, col_up + lp.n);
+    lp.row_lo.assign(row_lo, row_lo + lp.m), lp.row_up.assign(row_up, row_up + lp.m);
+    scale_vectors(P_, lp.cost.data(), lp.col_lo.data(), lp.col_up.data(), lp.row_lo.data(), lp.row_up.data(),
+                  lp.maximize ? -1.0 : 1.0);
+    if (fast_) fast_->set_vectors(P_.c.data(), P_.l.data(), P_.u.data(), P_.rl.data(), P_.ru.data());
+    else dev_->set_vectors(P_.c.data(), P_.l.data(), P_.u.data(), P_.rl.data(), P_.ru.data());
+}
+
 Report solve_lp(const LP& lp, const Options& o, Timings tm, Clock::time_point t_start, const double* x0_orig,
                 const double* y0_orig) {
     Session s(lp, o, tm);

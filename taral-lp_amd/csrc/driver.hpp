@@ -94,6 +94,10 @@ class Session {
 public:
     Session(const LP& lp, const Options& o, Timings tm = Timings());
     Report solve(const double* x0_orig = nullptr, const double* y0_orig = nullptr);
+    // New cost and bounds (original space) for the same A: rescaled with the session's factors and uploaded.
+    // `lp` must be the model the session was created on; its vectors are overwritten.
+    void set_vectors(LP& lp, const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                     const double* row_up);
     const Scaled& scaled() const { return P_; }
     Device* device() { return dev_.get(); }
     FastDevice* fast_device() { return fast_.get(); }

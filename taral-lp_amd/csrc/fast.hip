@@ -315,6 +315,13 @@ public:
         HIP_OK(hipGetLastError());
     }
 
+    void set_vectors(const double* c_, const double* l_, const double* u_, const double* rl_, const double* ru_) override {
+        auto put = [&](double* dst, const double* src, int k) {
+            if (k) HIP_OK(hipMemcpyAsync(dst, src, size_t(k) * sizeof(double), hipMemcpyHostToDevice, st_));
+        };
+        put(c64, c_, n_), put(l64, l_, n_), put(u64, u_, n_), put(rl64, rl_, m_), put(ru64, ru_, m_);
+        HIP_OK(hipStreamSynchronize(st_));
+    }
     void set_profile(bool on) override { profile_ = on; }
     KernelTimes kernel_times() override {
         HIP_OK(hipStreamSynchronize(st_));

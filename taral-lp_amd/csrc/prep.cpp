@@ -33,6 +33,25 @@ double row_bound_measure(double lo, double up) {
     return 0;
 }
 
+void scale_vectors(Scaled& P, const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                   const double* row_up, double sign) {
+    const int m = P.m, n = P.n;
+    P.c.resize(size_t(n)), P.l.resize(size_t(n)), P.u.resize(size_t(n));
+    P.rl.resize(size_t(m)), P.ru.resize(size_t(m));
+    double cn = 0, bn = 0;
+    for (int j = 0; j < n; ++j) {
+        const double c = sign * cost[j], d = P.Dc[size_t(j)];
+        cn += c * c;
+        P.c[size_t(j)] = c * d, P.l[size_t(j)] = col_lo[j] / d, P.u[size_t(j)] = col_up[j] / d;
+    }
+    for (int i = 0; i < m; ++i) {
+        const double b = row_bound_measure(row_lo[i], row_up[i]);
+        bn += b * b;
+        P.rl[size_t(i)] = row_lo[i] * P.Dr[size_t(i)], P.ru[size_t(i)] = row_up[i] * P.Dr[size_t(i)];
+    }
+    P.bnorm = std::sqrt(bn), P.cnorm = std::sqrt(cn);
+}
+
 namespace {
 
 // Divides A by sqrt of the row and column measures and accumulates the factors (gpu/pdhg.cu:95-102).
@@ -74,20 +93,7 @@ void finish_scaling(Scaled& P, const double* cost, const double* col_lo, const d
         }
     divide_by_measures(P, rs, cs);
     P.AT = transpose(A, n);
-    P.c.resize(size_t(n)), P.l.resize(size_t(n)), P.u.resize(size_t(n));
-    P.rl.resize(size_t(m)), P.ru.resize(size_t(m));
-    double cn = 0, bn = 0;
-    for (int j = 0; j < n; ++j) {
-        const double c = sign * cost[j], d = P.Dc[size_t(j)];
-        cn += c * c;
-        P.c[size_t(j)] = c * d, P.l[size_t(j)] = col_lo[j] / d, P.u[size_t(j)] = col_up[j] / d;
-    }
-    for (int i = 0; i < m; ++i) {
-        const double b = row_bound_measure(row_lo[i], row_up[i]);
-        bn += b * b;
-        P.rl[size_t(i)] = row_lo[i] * P.Dr[size_t(i)], P.ru[size_t(i)] = row_up[i] * P.Dr[size_t(i)];
-    }
-    P.bnorm = std::sqrt(bn), P.cnorm = std::sqrt(cn);
+    scale_vectors(P, cost, col_lo, col_up, row_lo, row_up, sign);
 }
 
 }  // namespace

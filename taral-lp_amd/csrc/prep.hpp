@@ -35,6 +35,11 @@ Scaled build_scaled_csc(int m, int n, const int64_t* colptr, const int32_t* rowi
                         const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
                         const double* row_up, double sign);
 
+// Scaled cost, bounds and original-space norms from the model vectors with the scaling factors P.Dr, P.Dc
+// already set (gpu/pdhg.cu:117-131); used again when a scenario changes only the vectors.
+void scale_vectors(Scaled& P, const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                   const double* row_up, double sign);
+
 // The reference's row measure b_i: the larger finite |row bound|, 0 when both are infinite.
 double row_bound_measure(double lo, double up);
 

@@ -33,7 +33,10 @@ EXPORTS = [
     "tpdhg_fast_set_steps", "tpdhg_fast_restart", "tpdhg_fast_iterate", "tpdhg_fast_eval", "tpdhg_fast_end_round",
     "tpdhg_fast_download", "tpdhg_fast_destroy",
     "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
+    "tpdhg_session_set_vectors",
     "tpdhg_result_json", "tpdhg_spmv_bench",
+    "tpdhg_mlp_create", "tpdhg_mlp_param_count", "tpdhg_mlp_set_params", "tpdhg_mlp_get_params",
+    "tpdhg_mlp_forward", "tpdhg_mlp_loss_grad", "tpdhg_mlp_train", "tpdhg_mlp_destroy",
 ]
 
 
@@ -139,8 +142,19 @@ def lib():
         L.tpdhg_session_create.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(C.c_void_p)]
         L.tpdhg_session_solve.argtypes = [C.c_void_p, _d, _d, C.POINTER(Result), _d, _d]
         L.tpdhg_session_free.argtypes = [C.c_void_p]
+        L.tpdhg_session_set_vectors.argtypes = [C.c_void_p, _d, _d, _d, _d, _d]
         L.tpdhg_result_json.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result), C.c_char_p, C.c_int32]
         L.tpdhg_spmv_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d]
+        _f = C.POINTER(C.c_float)
+        L.tpdhg_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]
+        L.tpdhg_mlp_param_count.argtypes = [C.c_void_p]
+        L.tpdhg_mlp_param_count.restype = C.c_int64
+        L.tpdhg_mlp_set_params.argtypes = [C.c_void_p, _f]
+        L.tpdhg_mlp_get_params.argtypes = [C.c_void_p, _f]
+        L.tpdhg_mlp_forward.argtypes = [C.c_void_p, _f, C.c_int32, _f]
+        L.tpdhg_mlp_loss_grad.argtypes = [C.c_void_p, _f, _f, C.c_int32, _d, _f]
+        L.tpdhg_mlp_train.argtypes = [C.c_void_p, _f, _f, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _d]
+        L.tpdhg_mlp_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -424,6 +438,11 @@ class Session:
         _check(lib().tpdhg_session_solve(self._h, _p(x0), _p(y0), C.byref(r), _p(x), _p(y)))
         return _result_dict(self.model, self.opts, r, x, y, self.m, self.n)
 
+    def set_vectors(self, cost, col_lo, col_up, row_lo, row_up):
+        """New cost and bounds (original space) for the same A (scenario families)."""
+        keep = [_f64(v) for v in (cost, col_lo, col_up, row_lo, row_up)]
+        _check(lib().tpdhg_session_set_vectors(self._h, *[_p(v) for v in keep]))
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value and _lib is not None:
             _lib.tpdhg_session_free(self._h)
@@ -443,6 +462,77 @@ def solve(model: Model, x0=None, y0=None, **opts) -> dict:
     x0, y0 = _f64(x0), _f64(y0)
     _check(lib().tpdhg_solve(model._h, C.byref(o), _p(x0), _p(y0), C.byref(r), _p(x), _p(y)))
     return _result_dict(model, o, r, x, y, i.m, i.n)
+
+
+_f32p = C.POINTER(C.c_float)
+
+
+def _f32(a, cols):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != cols:
+        raise ValueError(f"expected a 2-D array with {cols} columns, got shape {a.shape}")
+    return a
+
+
+class Mlp:
+    """Warm-start network on the GPU: n_in -> h1 -> h2 -> n_out, ReLU, bf16 MFMA products, FP32 master weights.
+    Parameters are one FP32 vector in the order W1 [n_in x h1], b1, W2 [h1 x h2], b2, W3 [h2 x n_out], b3."""
+
+    def __init__(self, n_in, h1, h2, n_out, seed=0):
+        self.shape = (int(n_in), int(h1), int(h2), int(n_out))
+        h = C.c_void_p()
+        _check(lib().tpdhg_mlp_create(*self.shape, int(seed), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().tpdhg_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def param_count(self) -> int:
+        return int(lib().tpdhg_mlp_param_count(self._h))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.param_count(), dtype=np.float32)
+        _check(lib().tpdhg_mlp_get_params(self._h, p.ctypes.data_as(_f32p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.param_count():
+            raise ValueError(f"expected {self.param_count()} parameters, got {p.size}")
+        _check(lib().tpdhg_mlp_set_params(self._h, p.ctypes.data_as(_f32p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x, self.shape[0])
+        y = np.empty((x.shape[0], self.shape[3]), dtype=np.float32)
+        _check(lib().tpdhg_mlp_forward(self._h, x.ctypes.data_as(_f32p), x.shape[0], y.ctypes.data_as(_f32p)))
+        return y
+
+    def loss_grad(self, x, t):
+        x, t = _f32(x, self.shape[0]), _f32(t, self.shape[3])
+        if x.shape[0] != t.shape[0]:
+            raise ValueError("x and t differ in rows")
+        g = np.empty(self.param_count(), dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().tpdhg_mlp_loss_grad(self._h, x.ctypes.data_as(_f32p), t.ctypes.data_as(_f32p), x.shape[0],
+                                         C.byref(loss), g.ctypes.data_as(_f32p)))
+        return loss.value, g
+
+    def train(self, x, t, batch=256, epochs=10, lr=1e-3, seed=0) -> float:
+        x, t = _f32(x, self.shape[0]), _f32(t, self.shape[3])
+        if x.shape[0] != t.shape[0]:
+            raise ValueError("x and t differ in rows")
+        loss = C.c_double()
+        _check(lib().tpdhg_mlp_train(self._h, x.ctypes.data_as(_f32p), t.ctypes.data_as(_f32p), x.shape[0],
+                                     int(batch), int(epochs), float(lr), int(seed), C.byref(loss)))
+        return loss.value
 
 
 def spmv_bench(ptr, idx, val, v, precision_bits=32, reps=20):
