@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="solver progress on stderr every 50 evaluations")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying HIP graphs (used under rocprofv3)")
+    ap.add_argument("--progress", type=float, default=30.0,
+                    help="seconds between the solver's progress lines on stderr (0 = none)")
     ap.add_argument("--cpu-sample-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
@@ -126,7 +128,8 @@ def main():
 
     def timed(mode_name, steps, warmup, profile):
         opts = dict(tol=a.tol, time_limit=a.time_limit, max_iter=a.max_iter, profile_kernels=int(profile),
-                    use_graph=0 if a.no_graph else 1, verbose=int(a.verbose), **mode_options(mode_name))
+                    use_graph=0 if a.no_graph else 1, verbose=int(a.verbose), progress_interval=a.progress,
+                    **mode_options(mode_name))
         t0 = time.perf_counter()
         sess = ffi.Session(model, **opts)
         setup = time.perf_counter() - t0

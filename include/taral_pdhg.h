@@ -164,6 +164,10 @@ typedef struct {
     int32_t halpern;   /* 1: fast path, reflected Halpern PDHG on an FP64-recentred problem (no reference
                           counterpart; the reference algorithm is the default, halpern = 0) */
     double reflection; /* Halpern reflection coefficient in [1, 2) (default 1.9) */
+    double primal_weight; /* initial primal weight of a warm-started solve, 0 = ||c||/||b|| as for a cold start
+                             (no reference counterpart: the reference always cold-starts, gpu/pdhg.cu:450-458) */
+    int32_t warm_fallback; /* 1: keep a warm start only when its relative KKT is below the cold start's */
+    double progress_interval; /* > 0: one progress line on stderr every this many seconds of solving */
 } tpdhg_options;
 void tpdhg_options_default(tpdhg_options* o);
 
@@ -182,6 +186,8 @@ typedef struct {
     double row_kernel_ms, col_kernel_ms; /* HIP-event totals when profile_kernels */
     int64_t row_launches, col_launches;
     char device[64];
+    int32_t warm_start_used;  /* the solve started from the caller's x0/y0 (0 after a fallback or a cold start) */
+    double warm_kkt, cold_kkt; /* relative KKT of both starts when warm_fallback compared them, else 0 */
 } tpdhg_result;
 
 /* Scales, uploads and solves (gpu/pdhg.cu:913-935 -> finish -> run). x (n) receives the original-space point
@@ -202,6 +208,9 @@ int tpdhg_session_solve(tpdhg_session* s, const double* x0, const double* y0, tp
  * on A only) and uploaded. For scenario families that share A (no reference counterpart). */
 int tpdhg_session_set_vectors(tpdhg_session* s, const double* cost, const double* col_lo, const double* col_up,
                               const double* row_lo, const double* row_up);
+/* Initial primal weight for the session's following warm-started solves (0 = ||c||/||b||); the learned warm
+ * start predicts it with (x0, y0). No reference counterpart. */
+int tpdhg_session_set_primal_weight(tpdhg_session* s, double primal_weight);
 void tpdhg_session_free(tpdhg_session* s);
 
 /* The JSON object gpu/pdhg.cu:881-897 writes (same keys and formats, plus this build's keys). */

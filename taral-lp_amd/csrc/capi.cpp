@@ -74,6 +74,7 @@ tpdhg::Options to_options(const tpdhg_options* o) {
     r.inner_tol = o->inner_tol, r.max_refinements = o->max_refinements, r.graph = o->use_graph != 0;
     r.verbose = o->verbose != 0, r.profile_kernels = o->profile_kernels != 0;
     r.halpern = o->halpern != 0, r.rho = o->reflection;
+    r.omega0 = o->primal_weight, r.warm_fallback = o->warm_fallback != 0, r.progress_s = o->progress_interval;
     return r;
 }
 }  // namespace
@@ -310,6 +311,7 @@ void tpdhg_options_default(tpdhg_options* o) {
     o->inner_tol = d.inner_tol, o->max_refinements = d.max_refinements, o->use_graph = d.graph ? 1 : 0;
     o->verbose = 0, o->profile_kernels = 0;
     o->halpern = d.halpern ? 1 : 0, o->reflection = d.rho;
+    o->primal_weight = d.omega0, o->warm_fallback = d.warm_fallback ? 1 : 0, o->progress_interval = d.progress_s;
 }
 
 namespace {
@@ -330,6 +332,8 @@ void fill_result(const tpdhg::Report& r, const tpdhg::LP& lp, tpdhg_result* res,
     res->row_kernel_ms = r.out.kernels.row_ms, res->col_kernel_ms = r.out.kernels.col_ms;
     res->row_launches = r.out.kernels.row_launches, res->col_launches = r.out.kernels.col_launches;
     std::snprintf(res->device, sizeof res->device, "%s", r.device_name.c_str());
+    res->warm_start_used = r.out.warm_used ? 1 : 0;
+    res->warm_kkt = r.out.warm_kkt, res->cold_kkt = r.out.cold_kkt;
     if (x) std::memcpy(x, r.x.data(), r.x.size() * sizeof(double));
     if (y) std::memcpy(y, r.y.data(), r.y.size() * sizeof(double));
 }
@@ -338,6 +342,7 @@ int check_options(const tpdhg::Options& o) {
     if (o.precision_bits != 32 && o.precision_bits != 64) return fail(TPDHG_ERR_ARG, "precision_bits must be 32 or 64");
     if (o.refine && !o.halpern) return fail(TPDHG_ERR_ARG, "refine runs on the Halpern path: set halpern = 1");
     if (o.halpern && !(o.rho >= 1 && o.rho < 2)) return fail(TPDHG_ERR_ARG, "reflection must be in [1, 2)");
+    if (!(o.omega0 >= 0) || !std::isfinite(o.omega0)) return fail(TPDHG_ERR_ARG, "primal_weight must be finite and >= 0");
     return TPDHG_OK;
 }
 }  // namespace
@@ -381,6 +386,13 @@ int tpdhg_session_solve(tpdhg_session* s, const double* x0, const double* y0, tp
     });
 }
 
+int tpdhg_session_set_primal_weight(tpdhg_session* s, double w) {
+    if (!s) return fail(TPDHG_ERR_ARG, "null argument");
+    if (!(w >= 0) || !std::isfinite(w)) return fail(TPDHG_ERR_ARG, "primal_weight must be finite and >= 0");
+    s->s->set_omega0(w);
+    return TPDHG_OK;
+}
+
 int tpdhg_session_set_vectors(tpdhg_session* s, const double* cost, const double* col_lo, const double* col_up,
                               const double* row_lo, const double* row_up) {
     if (!s || !s->s || !cost || !col_lo || !col_up || !row_lo || !row_up) return fail(TPDHG_ERR_ARG, "null argument");
@@ -408,6 +420,7 @@ int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const 
         r.tm.parse = res->parse_s, r.tm.scale = res->scale_s, r.tm.init = res->device_init_s;
         r.tm.transfer = res->transfer_s;
         r.device_name = res->device, r.device_mem_bytes = size_t(res->device_mem_bytes), r.nnz = res->nnz;
+        r.out.warm_used = res->warm_start_used != 0;
         std::string s = tpdhg::report_json(r, model->lp, to_options(opt));
         std::snprintf(buf, size_t(len), "%s", s.c_str());
         return TPDHG_OK;
@@ -429,16 +442,25 @@ int64_t tpdhg_mlp_param_count(const tpdhg_mlp* net) { return net && net->net ? n
         if (!(net) || !(net)->net) return fail(TPDHG_ERR_ARG, "null network"); \
         return guarded([&]() -> int { body; return TPDHG_OK; });             \
     } while (0)
-int tpdhg_mlp_set_params(tpdhg_mlp* net, const float* p) { MLP_CALL(net, net->net->set_params(p)); }
-int tpdhg_mlp_get_params(tpdhg_mlp* net, float* p) { MLP_CALL(net, net->net->get_params(p)); }
+int tpdhg_mlp_set_params(tpdhg_mlp* net, const float* p) {
+    if (!p) return fail(TPDHG_ERR_ARG, "null argument");
+    MLP_CALL(net, net->net->set_params(p));
+}
+int tpdhg_mlp_get_params(tpdhg_mlp* net, float* p) {
+    if (!p) return fail(TPDHG_ERR_ARG, "null argument");
+    MLP_CALL(net, net->net->get_params(p));
+}
 int tpdhg_mlp_forward(tpdhg_mlp* net, const float* x, int32_t batch, float* y) {
+    if (!x || !y) return fail(TPDHG_ERR_ARG, "null argument");
     MLP_CALL(net, net->net->forward(x, batch, y));
 }
 int tpdhg_mlp_loss_grad(tpdhg_mlp* net, const float* x, const float* t, int32_t batch, double* loss, float* grad) {
+    if (!x || !t) return fail(TPDHG_ERR_ARG, "null argument");
     MLP_CALL(net, { const double l = net->net->loss_grad(x, t, batch, grad); if (loss) *loss = l; });
 }
 int tpdhg_mlp_train(tpdhg_mlp* net, const float* x, const float* t, int32_t n, int32_t batch, int32_t epochs, float lr,
                     uint64_t seed, double* loss) {
+    if (!x || !t) return fail(TPDHG_ERR_ARG, "null argument");
     MLP_CALL(net, { const double l = net->net->train(x, t, n, batch, epochs, lr, seed); if (loss) *loss = l; });
 }
 void tpdhg_mlp_destroy(tpdhg_mlp* net) { delete net; }

@@ -18,6 +18,21 @@ struct Rel {
     double p, d, g;
     double worst() const { return std::max(p, std::max(d, g)); }
 };
+
+// True once every o.progress_s seconds (never when 0): the solve loop then prints a progress line on stderr.
+bool progress_due(const Options& o, Clock::time_point& last) {
+    if (!(o.progress_s > 0) || since(last) < o.progress_s) return false;
+    last = Clock::now();
+    return true;
+}
+
+// Original-space relative residuals of one evaluation (gpu/pdhg.cu:791-796).
+Rel rel_of(const double* k, const Scaled& P, double row_rel) {
+    const double po = k[POBJ], du = k[DROW] + k[DCOL];
+    const double prow = row_rel > 0 ? std::sqrt(k[PR2R]) / row_rel : 0;
+    return Rel{std::max(std::sqrt(k[PR2O]) / (1 + P.bnorm), prow), std::sqrt(k[DR2O]) / (1 + P.cnorm),
+               std::fabs(po - du) / (1 + std::fabs(po) + std::fabs(du))};
+}
 }  // namespace
 
 LP flatten(const Model& md) {
@@ -48,7 +63,7 @@ const char* input_error(const LP& lp) {
 Outcome run_pdhg(Device& b, const Scaled& P, const Options& o, double time_budget, const double* x0, const double* z0,
                  double omega0, double norm_a) {
     Outcome r;
-    auto t0 = Clock::now();
+    auto t0 = Clock::now(), t_progress = t0;
     b.set_profile(o.profile_kernels);
     r.norm_a = norm_a > 0 ? norm_a : b.power_norm(200);
     r.power_s = since(t0);
@@ -69,12 +84,7 @@ Outcome run_pdhg(Device& b, const Scaled& P, const Options& o, double time_budge
         const double g = k[POBJ] - (k[DROW] + k[DCOL]);
         return std::sqrt(w * k[PR2S] + k[DR2S] / w + g * g);
     };
-    auto rel = [&](const double* k) {  // original-space relative residuals (gpu/pdhg.cu:791-796)
-        const double po = k[POBJ], du = k[DROW] + k[DCOL];
-        const double prow = o.row_rel > 0 ? std::sqrt(k[PR2R]) / o.row_rel : 0;
-        return Rel{std::max(std::sqrt(k[PR2O]) / (1 + P.bnorm), prow), std::sqrt(k[DR2O]) / (1 + P.cnorm),
-                   std::fabs(po - du) / (1 + std::fabs(po) + std::fabs(du))};
-    };
+    auto rel = [&](const double* k) { return rel_of(k, P, o.row_rel); };
     double kc[NKKT], ka[NKKT];
     b.eval2(1.0, kc, nullptr);
     double k_last = wkkt(kc), k_prev = kInf;
@@ -91,7 +101,7 @@ Outcome run_pdhg(Device& b, const Scaled& P, const Options& o, double time_budge
         const Rel rc = rel(kc), ra = rel(ka);
         const bool use_avg = ra.worst() < rc.worst();
         best = use_avg ? ra : rc, avg_final = use_avg;
-        if (o.verbose && (it / o.eval_freq) % 50 == 0)
+        if ((o.verbose && (it / o.eval_freq) % 50 == 0) || progress_due(o, t_progress))
             std::fprintf(stderr, "it %ld  w %.3e  cur %.2e %.2e %.2e  avg %.2e %.2e %.2e  pobj %.10g\n", long(it), w,
                          rc.p, rc.d, rc.g, ra.p, ra.d, ra.g, kc[POBJ]);
         if (!std::isfinite(rc.worst()) && !std::isfinite(ra.worst())) {
@@ -137,7 +147,7 @@ Outcome run_pdhg(Device& b, const Scaled& P, const Options& o, double time_budge
 Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_budget, const double* x0, const double* z0,
                  double omega0, double norm_a) {
     Outcome r;
-    auto t0 = Clock::now();
+    auto t0 = Clock::now(), t_progress = t0;
     b.set_profile(o.profile_kernels);
     r.norm_a = norm_a > 0 ? norm_a : b.power_norm(200);
     r.power_s = since(t0);
@@ -156,12 +166,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
         const double g = k[POBJ] - (k[DROW] + k[DCOL]);
         return std::sqrt(w * k[PR2S] + k[DR2S] / w + g * g);
     };
-    auto rel = [&](const double* k) {
-        const double po = k[POBJ], du = k[DROW] + k[DCOL];
-        const double prow = o.row_rel > 0 ? std::sqrt(k[PR2R]) / o.row_rel : 0;
-        return Rel{std::max(std::sqrt(k[PR2O]) / (1 + P.bnorm), prow), std::sqrt(k[DR2O]) / (1 + P.cnorm),
-                   std::fabs(po - du) / (1 + std::fabs(po) + std::fabs(du))};
-    };
+    auto rel = [&](const double* k) { return rel_of(k, P, o.row_rel); };
     double kk[NKKT];
     int64_t it = 0;
     Rel best{kInf, kInf, kInf};
@@ -192,7 +197,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
             b.eval(kk);
             const Rel rc = rel(kk);
             best = rc;
-            if (o.verbose && (it / o.eval_freq) % 50 == 0)
+            if ((o.verbose && (it / o.eval_freq) % 50 == 0) || progress_due(o, t_progress))
                 std::fprintf(stderr, "it %ld  round %d  w %.3e  p %.2e d %.2e g %.2e  pobj %.10g\n", long(it), round, w,
                              rc.p, rc.d, rc.g, kk[POBJ]);
             if (!std::isfinite(rc.worst())) {
@@ -310,12 +315,21 @@ Report Session::solve(const double* x0_orig, const double* y0_orig) {
         zs0.resize(size_t(lp.m));
         for (int i = 0; i < lp.m; ++i) zs0[size_t(i)] = y0_orig[i] / P.Dr[size_t(i)];
     }
+    const double* xw = x0_orig ? xs0.data() : nullptr;
+    const double* zw = y0_orig ? zs0.data() : nullptr;
+    bool warm = xw || zw;
+    double warm_kkt = 0, cold_kkt = 0;
+    if (warm && o_.warm_fallback) {  // keep the warm start only when its relative KKT beats the cold start's
+        warm_kkt = probe_kkt(xw, zw);
+        cold_kkt = probe_kkt(nullptr, nullptr);
+        if (!(warm_kkt < cold_kkt)) warm = false, xw = zw = nullptr;
+    }
     if (fast_)
-        rep.out = run_fast(*fast_, P, o_, o_.time_limit - setup_s_, x0_orig ? xs0.data() : nullptr,
-                           y0_orig ? zs0.data() : nullptr);
+        rep.out = run_fast(*fast_, P, o_, o_.time_limit - setup_s_, xw, zw, warm ? o_.omega0 : 0, norm_a_);
     else
-        rep.out = run_pdhg(*dev_, P, o_, o_.time_limit - setup_s_, x0_orig ? xs0.data() : nullptr,
-                           y0_orig ? zs0.data() : nullptr);
+        rep.out = run_pdhg(*dev_, P, o_, o_.time_limit - setup_s_, xw, zw, warm ? o_.omega0 : 0, norm_a_);
+    norm_a_ = rep.out.norm_a;  // A_s is fixed for the session's lifetime: later solves skip the power iteration
+    rep.out.warm_used = warm, rep.out.warm_kkt = warm_kkt, rep.out.cold_kkt = cold_kkt;
     rep.solve_s = since(t_call);
     // Unscale and check the point on the ORIGINAL model in FP64 (gpu/pdhg.cu:866-876).
     rep.x.resize(size_t(lp.n)), rep.y.resize(size_t(lp.m));
@@ -336,6 +350,19 @@ Report Session::solve(const double* x0_orig, const double* y0_orig) {
     rep.objective = double(obj), rep.max_row_violation = rviol, rep.max_bound_violation = bviol;
     rep.total_s = setup_s_ + since(t_call);
     return rep;
+}
+
+double Session::probe_kkt(const double* xs, const double* zs) {
+    double kk[NKKT];
+    if (fast_) {
+        fast_->set_base(xs, zs);
+        fast_->begin_round();
+        fast_->eval(kk);
+    } else {
+        dev_->start(xs, zs);
+        dev_->eval2(1.0, kk, nullptr);
+    }
+    return rel_of(kk, P_, o_.row_rel).worst();
 }
 
 void Session::set_vectors(LP& lp, const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
@@ -371,7 +398,8 @@ std::string report_json(const Report& r, const LP& lp, const Options& o) {
         "\"solve_s\": %.6f, \"power_iter_s\": %.6f, \"total_s\": %.6f, \"device\": \"%s\", \"backend\": \"%s\", "
         "\"threads\": %d, \"precision\": \"%s\", \"rows\": %d, \"cols\": %d, \"nnz\": %ld, \"norm_a\": %.6e, "
         "\"primal_weight\": %.6e, \"eval_freq\": %d, \"graph\": %s, \"device_mem_bytes\": %zu, "
-        "\"precision_mode\": \"%s\", \"refinements\": %d, \"iters_per_s\": %.6e, \"solves_per_s\": %.6e}\n",
+        "\"precision_mode\": \"%s\", \"refinements\": %d, \"iters_per_s\": %.6e, \"solves_per_s\": %.6e, "
+        "\"warm_start_used\": %s}\n",
         q.status.c_str(), r.objective, q.rel_primal, q.rel_dual, q.rel_gap, o.tol, long(q.iterations), q.restarts,
         q.from_average ? "average" : "current", r.max_row_violation, r.max_bound_violation, r.setup_s, r.tm.parse,
         r.tm.scale, r.tm.init, r.tm.transfer, r.solve_s, q.power_s, r.total_s, r.device_name.c_str(), "gpu", 0,
@@ -381,7 +409,7 @@ std::string report_json(const Report& r, const LP& lp, const Options& o) {
          (o.precision_bits == 32 ? (o.refine ? "fp32+fp64-refinement" : "fp32") : (o.refine ? "fp64+refinement" : "fp64")))
             .c_str(),
         q.refinements, iters_per_s,
-        r.solve_s > 0 ? 1.0 / r.solve_s : 0.0);
+        r.solve_s > 0 ? 1.0 / r.solve_s : 0.0, q.warm_used ? "true" : "false");
     return buf;
 }
 

@@ -45,6 +45,9 @@ struct Options {  // defaults of gpu/pdhg.cu:744-750 plus this build's additions
     bool profile_kernels = false;
     bool halpern = false;      // fast path: reflected Halpern PDHG on the FP64-recentred problem
     double rho = 1.9;          // its reflection coefficient
+    double omega0 = 0;         // initial primal weight of a warm-started solve (0: ||c||/||b|| as for a cold start)
+    bool warm_fallback = false;  // cold-start instead when the warm point's relative KKT is not below the cold one's
+    double progress_s = 0;       // > 0: a progress line on stderr every progress_s seconds of solving
 };
 
 struct Outcome {
@@ -55,6 +58,8 @@ struct Outcome {
     double rel_primal = 0, rel_dual = 0, rel_gap = 0, norm_a = 0, omega = 1, power_s = 0, iterate_s = 0;
     std::vector<double> x, z;  // scaled-space point
     KernelTimes kernels;
+    bool warm_used = false;            // the solve started from the caller's point
+    double warm_kkt = 0, cold_kkt = 0;  // relative KKT of the warm and the cold start when the fallback compared them
 };
 
 // Restarted averaged PDHG on the device backend (gpu/pdhg.cu:761-849). x0/z0 (scaled space, nullable) warm start
@@ -89,11 +94,14 @@ struct Report {  // what the CLI prints and the C ABI returns (gpu/pdhg.cu:866-8
 };
 
 // A problem resident on the device: scaled once, uploaded once, then solved any number of times from cold or
-// warm starts (each solve runs the power iteration and the PDHG loop afresh).
+// warm starts. The first solve runs the power iteration; later solves reuse ||A_s|| (A_s never changes).
 class Session {
 public:
     Session(const LP& lp, const Options& o, Timings tm = Timings());
+    // Warm start from (x0, y0) (original space, nullable) with Options::omega0; with Options::warm_fallback the
+    // start is kept only when its relative KKT is below the cold start's.
     Report solve(const double* x0_orig = nullptr, const double* y0_orig = nullptr);
+    void set_omega0(double w) { o_.omega0 = w; }
     // New cost and bounds (original space) for the same A: rescaled with the session's factors and uploaded.
     // `lp` must be the model the session was created on; its vectors are overwritten.
     void set_vectors(LP& lp, const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
@@ -110,7 +118,8 @@ private:
     Scaled P_;
     std::unique_ptr<Device> dev_;
     std::unique_ptr<FastDevice> fast_;
-    double setup_s_ = 0;
+    double probe_kkt(const double* xs, const double* zs);  // relative KKT at a scaled-space start point
+    double setup_s_ = 0, norm_a_ = 0;
     std::chrono::steady_clock::time_point created_;
 };
 

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <memory>
 #include <random>
+#include <stdexcept>
 
 #include "hipcommon.hpp"
 #include "mlp.hpp"
@@ -132,9 +133,9 @@ int pad16(int k) { return (k + 15) / 16 * 16; }
 class MlpImpl final : public Mlp {
 public:
     MlpImpl(int n_in, int h1, int h2, int n_out, uint64_t seed) : in_(n_in), h1_(h1), h2_(h2), out_(n_out) {
-        require_device();
         if (n_in % 16 || h1 % 16 || h2 % 16 || n_in <= 0 || h1 <= 0 || h2 <= 0 || n_out <= 0)
-            throw std::runtime_error("MLP widths: n_in, h1, h2 must be positive multiples of 16");
+            throw std::invalid_argument("MLP widths: n_in, h1, h2 must be positive multiples of 16, n_out positive");
+        require_device();
         op_ = pad16(n_out);
         HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         // parameter block (FP32): W1 [in x h1], b1, W2 [h1 x h2], b2, W3 [h2 x op], b3 [op]; padding columns of W3, b3
@@ -207,12 +208,17 @@ public:
     }
 
     double train(const float* x, const float* t, int n, int batch, int epochs, float lr, uint64_t seed) override {
-        if (n <= 0 || batch <= 0) throw std::runtime_error("train: empty data");
+        if (n <= 0 || batch <= 0 || epochs < 0) throw std::invalid_argument("train: empty data or negative epochs");
         batch = std::min(batch, n);
         ensure_batch(batch);
-        float* dx = mem_.alloc<float>(size_t(n) * in_);
-        float* dt = mem_.alloc<float>(size_t(n) * out_);
-        int* didx = mem_.alloc<int>(size_t(n));
+        if (n > data_cap_) {  // training set resident on the device, kept for later calls
+            data_cap_ = n;
+            dx_ = mem_.alloc<float>(size_t(n) * in_), dt_ = mem_.alloc<float>(size_t(n) * out_);
+            didx_ = mem_.alloc<int>(size_t(n));
+        }
+        float* dx = dx_;
+        float* dt = dt_;
+        int* didx = didx_;
         HIP_OK(hipMemcpyAsync(dx, x, size_t(n) * in_ * sizeof(float), hipMemcpyHostToDevice, st_));
         HIP_OK(hipMemcpyAsync(dt, t, size_t(n) * out_ * sizeof(float), hipMemcpyHostToDevice, st_));
         std::mt19937_64 rng(seed);
@@ -239,7 +245,7 @@ public:
 
 private:
     void ensure_batch(int batch) {
-        if (batch <= 0) throw std::runtime_error("batch must be positive");
+        if (batch <= 0) throw std::invalid_argument("batch must be positive");
         const int bp = pad16(batch);
         if (bp == bp_) return;
         if (bp > cap_) {
@@ -350,6 +356,9 @@ private:
     uint16_t *Xb_ = nullptr, *H1b_ = nullptr, *H2b_ = nullptr, *dYb_ = nullptr, *dZb_ = nullptr;
     float *Z1_ = nullptr, *Z2_ = nullptr, *Y_ = nullptr, *T_ = nullptr, *dY_ = nullptr, *dH_ = nullptr, *dZ_ = nullptr;
     float* xin_ = nullptr;
+    float *dx_ = nullptr, *dt_ = nullptr;
+    int* didx_ = nullptr;
+    int data_cap_ = 0;
     double *part_ = nullptr, *out_d_ = nullptr;
 };
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "tpdhg_fast_set_steps", "tpdhg_fast_restart", "tpdhg_fast_iterate", "tpdhg_fast_eval", "tpdhg_fast_end_round",
     "tpdhg_fast_download", "tpdhg_fast_destroy",
     "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
-    "tpdhg_session_set_vectors",
+    "tpdhg_session_set_vectors", "tpdhg_session_set_primal_weight",
     "tpdhg_result_json", "tpdhg_spmv_bench",
     "tpdhg_mlp_create", "tpdhg_mlp_param_count", "tpdhg_mlp_set_params", "tpdhg_mlp_get_params",
     "tpdhg_mlp_forward", "tpdhg_mlp_loss_grad", "tpdhg_mlp_train", "tpdhg_mlp_destroy",
@@ -69,7 +69,8 @@ class Options(C.Structure):
                 ("eval_freq", C.c_int32), ("row_rel", C.c_double), ("precision_bits", C.c_int32),
                 ("refine", C.c_int32), ("inner_tol", C.c_double), ("max_refinements", C.c_int32),
                 ("use_graph", C.c_int32), ("verbose", C.c_int32), ("profile_kernels", C.c_int32),
-                ("halpern", C.c_int32), ("reflection", C.c_double)]
+                ("halpern", C.c_int32), ("reflection", C.c_double), ("primal_weight", C.c_double),
+                ("warm_fallback", C.c_int32), ("progress_interval", C.c_double)]
 
 
 class Result(C.Structure):
@@ -82,7 +83,8 @@ class Result(C.Structure):
                 ("power_iter_s", C.c_double), ("total_s", C.c_double), ("norm_a", C.c_double),
                 ("primal_weight", C.c_double), ("device_mem_bytes", C.c_int64), ("m", C.c_int32), ("n", C.c_int32),
                 ("nnz", C.c_int64), ("row_kernel_ms", C.c_double), ("col_kernel_ms", C.c_double),
-                ("row_launches", C.c_int64), ("col_launches", C.c_int64), ("device", C.c_char * 64)]
+                ("row_launches", C.c_int64), ("col_launches", C.c_int64), ("device", C.c_char * 64),
+                ("warm_start_used", C.c_int32), ("warm_kkt", C.c_double), ("cold_kkt", C.c_double)]
 
 
 _lib = None
@@ -143,6 +145,7 @@ def lib():
         L.tpdhg_session_solve.argtypes = [C.c_void_p, _d, _d, C.POINTER(Result), _d, _d]
         L.tpdhg_session_free.argtypes = [C.c_void_p]
         L.tpdhg_session_set_vectors.argtypes = [C.c_void_p, _d, _d, _d, _d, _d]
+        L.tpdhg_session_set_primal_weight.argtypes = [C.c_void_p, C.c_double]
         L.tpdhg_result_json.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result), C.c_char_p, C.c_int32]
         L.tpdhg_spmv_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d]
         _f = C.POINTER(C.c_float)
@@ -431,7 +434,10 @@ class Session:
         _check(lib().tpdhg_session_create(model._h, C.byref(self.opts), C.byref(h)))
         self._h = h
 
-    def solve(self, x0=None, y0=None) -> dict:
+    def solve(self, x0=None, y0=None, primal_weight=None) -> dict:
+        """primal_weight: initial primal weight of this and later warm-started solves (0 = default)."""
+        if primal_weight is not None:
+            _check(lib().tpdhg_session_set_primal_weight(self._h, float(primal_weight)))
         x, y = np.zeros(max(self.n, 1)), np.zeros(max(self.m, 1))
         r = Result()
         x0, y0 = _f64(x0), _f64(y0)

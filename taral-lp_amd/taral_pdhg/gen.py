@@ -255,3 +255,43 @@ def write_mps(model: LPModel, path: str) -> None:
             if up != np.inf:
                 f.write(f' UP BND X{j} {_r(up)}\n')
         f.write('ENDATA\n')
+
+
+class ScenarioFamily:
+    """Scenario LPs that share the base LP's matrix: each of ``n_factors`` factors owns a fixed random subset of
+    ``frac`` of the rows and of the columns, and a scenario theta (one value per factor) multiplies the row bounds
+    of its rows and the costs of its columns by exp(theta_k) (overlapping subsets multiply). The base rows of the
+    planted family are equalities, so a scenario moves b = row_lo = row_up; the planted optimum no longer holds.
+    """
+
+    def __init__(self, base: LPModel, n_factors: int = 32, frac: float = 0.0015, seed: int = 0):
+        rng = np.random.default_rng(seed)
+        self.base = base
+        self.n_factors = int(n_factors)
+        kr, kc = max(1, int(round(frac * base.m))), max(1, int(round(frac * base.n)))
+        self.row_sets = [rng.choice(base.m, kr, replace=False) for _ in range(self.n_factors)]
+        self.col_sets = [rng.choice(base.n, kc, replace=False) for _ in range(self.n_factors)]
+
+    def thetas(self, count: int, seed: int, sigma: float = 0.1) -> np.ndarray:
+        """``count`` scenarios, theta_k ~ N(0, sigma^2) i.i.d."""
+        return np.random.default_rng(seed).normal(0.0, sigma, size=(int(count), self.n_factors))
+
+    def vectors(self, theta):
+        """(cost, col_lo, col_up, row_lo, row_up) of scenario ``theta``."""
+        theta = np.asarray(theta, dtype=np.float64)
+        if theta.shape != (self.n_factors,):
+            raise ValueError(f"theta must have {self.n_factors} entries")
+        rs, cs = np.ones(self.base.m), np.ones(self.base.n)
+        for k in range(self.n_factors):
+            e = np.exp(theta[k])
+            rs[self.row_sets[k]] *= e
+            cs[self.col_sets[k]] *= e
+        b = self.base
+        return b.cost * cs, b.col_lo.copy(), b.col_up.copy(), b.row_lo * rs, b.row_up * rs
+
+    def model(self, theta, name: str = "SCENARIO") -> LPModel:
+        """The scenario as a stand-alone LPModel (same matrix arrays)."""
+        c, l, u, rl, ru = self.vectors(theta)
+        b = self.base
+        return LPModel(m=b.m, n=b.n, colptr=b.colptr, rowidx=b.rowidx, val=b.val, cost=c, col_lo=l, col_up=u,
+                       row_lo=rl, row_up=ru, name=name)

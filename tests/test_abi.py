@@ -100,3 +100,19 @@ def test_session_refuses_without_device_or_bad_options():
     with pytest.raises(ffi.TpdhgError) as e:
         ffi.Session(md, precision_bits=64, refine=1)
     assert e.value.code == ffi.ERR_ARG
+
+
+def test_mlp_create_checks_widths_then_device():
+    with pytest.raises(ffi.TpdhgError) as e:
+        ffi.Mlp(30, 512, 512, 257)
+    assert e.value.code == ffi.ERR_ARG and "multiples of 16" in str(e.value)
+    if ffi.device_count() == 0:
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.Mlp(32, 512, 512, 257)
+        assert e.value.code == ffi.ERR_NO_DEVICE
+
+
+def test_options_carry_warm_start_fields():
+    o = ffi.default_options()
+    assert o.primal_weight == 0 and o.warm_fallback == 0
+    assert ffi.Result().warm_start_used == 0
