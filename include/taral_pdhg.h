@@ -155,9 +155,10 @@ typedef struct {
     int32_t eval_freq; /* iterations per evaluation (default 64) */
     double row_rel;    /* per-row relative primal test factor, 0 = off (gpu/pdhg.cu:748) */
     int32_t precision_bits; /* 64 (default) or 32 */
-    int32_t refine;    /* 1: FP64 iterative refinement rounds around the inner solves (needs halpern = 1) */
-    double inner_tol;  /* relative KKT target of each refinement round (default 1e-4) */
-    int32_t max_refinements;
+    int32_t refine;    /* 1: FP64 iterative refinement (needs halpern = 1): every restart recentres the FP32 inner
+                          problem at the restart point with exact FP64 products */
+    double inner_tol;  /* ignored (kept for the struct layout) */
+    int32_t max_refinements; /* cap on the recentrings (default 1e9) */
     int32_t use_graph; /* capture each eval_freq-iteration block in a HIP graph (default 1) */
     int32_t verbose;
     int32_t profile_kernels; /* HIP-event timing of the row/column kernels */

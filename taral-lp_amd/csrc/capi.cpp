@@ -71,7 +71,7 @@ tpdhg::Options to_options(const tpdhg_options* o) {
     if (!o) return r;
     r.tol = o->tol, r.time_limit = o->time_limit, r.max_iter = o->max_iter, r.eval_freq = o->eval_freq;
     r.row_rel = o->row_rel, r.precision_bits = o->precision_bits, r.refine = o->refine != 0;
-    r.inner_tol = o->inner_tol, r.max_refinements = o->max_refinements, r.graph = o->use_graph != 0;
+    r.max_refinements = o->max_refinements, r.graph = o->use_graph != 0;
     r.verbose = o->verbose != 0, r.profile_kernels = o->profile_kernels != 0;
     r.halpern = o->halpern != 0, r.rho = o->reflection;
     r.omega0 = o->primal_weight, r.warm_fallback = o->warm_fallback != 0, r.progress_s = o->progress_interval;
@@ -308,7 +308,7 @@ void tpdhg_options_default(tpdhg_options* o) {
     tpdhg::Options d;
     o->tol = d.tol, o->time_limit = d.time_limit, o->max_iter = d.max_iter, o->eval_freq = d.eval_freq;
     o->row_rel = d.row_rel, o->precision_bits = d.precision_bits, o->refine = d.refine ? 1 : 0;
-    o->inner_tol = d.inner_tol, o->max_refinements = d.max_refinements, o->use_graph = d.graph ? 1 : 0;
+    o->inner_tol = 0, o->max_refinements = d.max_refinements, o->use_graph = d.graph ? 1 : 0;
     o->verbose = 0, o->profile_kernels = 0;
     o->halpern = d.halpern ? 1 : 0, o->reflection = d.rho;
     o->primal_weight = d.omega0, o->warm_fallback = d.warm_fallback ? 1 : 0, o->progress_interval = d.progress_s;

@@ -5,7 +5,7 @@
 // message, after the input gate, instead of running a CPU solver.
 //
 // Additional flags: --fast (the fast path: FP32 reflected Halpern PDHG iterates inside FP64 iterative
-// refinement = --fp32 --halpern --refine), --halpern, --reflection R, --refine, --inner-tol T,
+// refinement = --fp32 --halpern --refine), --halpern, --reflection R, --refine,
 // --max-refinements N, --profile-kernels, --warm-sol FILE / --warm-dual FILE (start point in the .sol/.dual
 // formats this program writes).
 #include <chrono>
@@ -75,7 +75,6 @@ int main(int argc, char** argv) {
         else if (a == "--halpern") o.halpern = true;
         else if (a == "--reflection") o.rho = std::atof(next());
         else if (a == "--fast") o.halpern = true, o.refine = true, o.precision_bits = 32;
-        else if (a == "--inner-tol") o.inner_tol = std::atof(next());
         else if (a == "--max-refinements") o.max_refinements = std::atoi(next());
         else if (a == "--profile-kernels") o.profile_kernels = true;
         else if (a == "--warm-sol") warm_sol = next();
@@ -86,7 +85,7 @@ int main(int argc, char** argv) {
         std::fprintf(stderr,
                      "usage: pdhg MODEL.mps --device gpu|cpu [--threads N] [--tol 1e-4] [--time-limit S] "
                      "[--max-iter N] [--json OUT.json] [--sol OUT.sol] [--dual OUT.dual] [--fp32] [--eval-freq 64] "
-                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--refine] [--inner-tol 1e-4] "
+                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--refine] "
                      "[--max-refinements 8] "
                      "[--profile-kernels] [--warm-sol X.sol] [--warm-dual X.dual]\n");
         return 2;

@@ -168,96 +168,72 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
     };
     auto rel = [&](const double* k) { return rel_of(k, P, o.row_rel); };
     double kk[NKKT];
-    int64_t it = 0;
-    Rel best{kInf, kInf, kInf};
-    bool done = false;
-    for (int round = 0; !done; ++round) {
-        const bool last_round = !o.refine || round >= o.max_refinements;
+    int64_t it = 0, n_since = 0;
+    b.begin_round();
+    b.set_steps(eta / w, eta * w);
+    b.restart();
+    b.eval(kk);
+    Rel best = rel(kk);
+    double k_last = wkkt(kk), k_prev = kInf;
+    // Each recentring folds the inner point into the FP64 base and rebuilds the inner problem with exact FP64
+    // products; the next inner solve then only spans one restart cycle, so its FP32 rounding stays relative to
+    // that cycle's (shrinking) distance instead of to the distance from the start.
+    auto recentre = [&]() {
+        b.end_round();
         b.begin_round();
         b.set_steps(eta / w, eta * w);
         b.restart();
+    };
+    if (best.worst() <= o.tol) r.status = "near_optimal";
+    while (r.status != "near_optimal") {
+        const int steps = int(std::min<int64_t>(o.eval_freq, o.max_iter - it));
+        if (steps <= 0) break;
+        b.iterate(steps);
+        it += steps, n_since += steps;
         b.eval(kk);
-        best = rel(kk);
-        const double start = best.worst();
-        if (start <= o.tol) {
-            r.status = "near_optimal";
+        const Rel rc = rel(kk);
+        best = rc;
+        if ((o.verbose && (it / o.eval_freq) % 50 == 0) || progress_due(o, t_progress))
+            std::fprintf(stderr, "it %ld  recentred %d  w %.3e  p %.2e d %.2e g %.2e  pobj %.10g\n", long(it),
+                         r.refinements, w, rc.p, rc.d, rc.g, kk[POBJ]);
+        if (!std::isfinite(rc.worst())) {
+            r.status = "numerical_failure";
             break;
         }
-        const double target = (!last_round && round == 0) ? std::max(o.tol, start * o.inner_tol) : o.tol;
-        double k_last = wkkt(kk), k_prev = kInf, best_round = start;
-        int64_t n_since = 0, round_it = 0, best_at = 0;
-        for (;;) {
-            const int steps = int(std::min<int64_t>(o.eval_freq, o.max_iter - it));
-            if (steps <= 0) {
-                done = true;
-                break;
-            }
-            b.iterate(steps);
-            it += steps, n_since += steps, round_it += steps;
+        if (rc.worst() <= o.tol) {
+            // Confirm with exact FP64 products: recentre at this point and evaluate there.
+            recentre();
             b.eval(kk);
-            const Rel rc = rel(kk);
-            best = rc;
-            if ((o.verbose && (it / o.eval_freq) % 50 == 0) || progress_due(o, t_progress))
-                std::fprintf(stderr, "it %ld  round %d  w %.3e  p %.2e d %.2e g %.2e  pobj %.10g\n", long(it), round, w,
-                             rc.p, rc.d, rc.g, kk[POBJ]);
-            if (!std::isfinite(rc.worst())) {
-                r.status = "numerical_failure";
-                done = true;
+            best = rel(kk);
+            if (best.worst() <= o.tol) {
+                r.status = "near_optimal";
                 break;
             }
-            if (rc.worst() <= o.tol) {
-                // Confirm in FP64: fold the inner point into the base and evaluate with exact FP64 products.
-                b.end_round();
-                b.begin_round();
+            r.refinements++;  // not yet: carry on from the recentred point
+            k_last = wkkt(kk), k_prev = kInf, n_since = 0;
+            continue;
+        }
+        if (since(t0) > time_budget) {
+            r.status = "time_limit";
+            break;
+        }
+        if (it >= o.max_iter) break;
+        // Restart tests of gpu/pdhg.cu:827-842 with the current point as the candidate.
+        const double kc = wkkt(kk);
+        const bool restart =
+            double(n_since) >= 0.36 * double(it) || kc <= 0.2 * k_last || (kc <= 0.8 * k_last && kc > k_prev);
+        k_prev = kc;
+        if (restart) {
+            const double dx = std::sqrt(kk[DX2]), dz = std::sqrt(kk[DZ2]);
+            if (dx > 1e-10 && dz > 1e-10) w = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w));
+            if (o.refine && r.refinements < o.max_refinements) {
+                recentre();
+                r.refinements++;
+            } else {
                 b.set_steps(eta / w, eta * w);
                 b.restart();
-                b.eval(kk);
-                best = rel(kk);
-                if (best.worst() <= o.tol) {
-                    r.status = "near_optimal";
-                    done = true;
-                    break;
-                }
-                r.refinements++;  // not yet: continue from this point as a new round
-                k_last = wkkt(kk), k_prev = kInf, n_since = 0, best_round = best.worst(), best_at = round_it;
-                continue;
             }
-            if (since(t0) > time_budget) {
-                r.status = "time_limit";
-                done = true;
-                break;
-            }
-            if (it >= o.max_iter) {
-                done = true;
-                break;
-            }
-            if (!last_round) {
-                if (rc.worst() <= target) break;  // recentre
-                if (rc.worst() < 0.5 * best_round) best_round = rc.worst(), best_at = round_it;
-                // Stalled: no halving for as many iterations as the last halving took (at least 8192) in the first
-                // round, whose FP32 products floor the evaluated gap; later rounds see FP64 base products and only
-                // recentre after twice that (at least 32768).
-                const int64_t wait = round == 0 ? std::max<int64_t>(8192, best_at) : std::max<int64_t>(32768, 2 * best_at);
-                if (round_it - best_at >= wait) break;
-            }
-            // Restart tests of gpu/pdhg.cu:827-842 with the current point as the candidate.
-            const double kc = wkkt(kk);
-            const bool restart =
-                double(n_since) >= 0.36 * double(round_it) || kc <= 0.2 * k_last || (kc <= 0.8 * k_last && kc > k_prev);
-            k_prev = kc;
-            if (restart) {
-                const double dx = std::sqrt(kk[DX2]), dz = std::sqrt(kk[DZ2]);
-                if (dx > 1e-10 && dz > 1e-10) {
-                    w = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w));
-                    b.set_steps(eta / w, eta * w);
-                }
-                b.restart();
-                k_last = wkkt(kk), k_prev = kInf, n_since = 0, r.restarts++;
-            }
-        }
-        if (!done) {
-            b.end_round();
-            r.refinements++;
+            k_last = wkkt(kk), k_prev = kInf, n_since = 0, r.restarts++;
         }
     }
     r.iterations = it;

@@ -39,9 +39,8 @@ struct Options {  // defaults of gpu/pdhg.cu:744-750 plus this build's additions
     double row_rel = 0;
     bool verbose = false, graph = true;
     int precision_bits = 64;   // 32: FP32 iterates and SpMV, FP64 reductions (the reference's --fp32)
-    bool refine = false;       // FP64 iterative refinement around FP32 inner solves
-    double inner_tol = 1e-4;   // relative KKT target of each refinement round's inner solve
-    int max_refinements = 8;
+    bool refine = false;       // FP64 iterative refinement: recentre the FP32 inner problem at every restart
+    int max_refinements = 1000000000;  // cap on the recentrings
     bool profile_kernels = false;
     bool halpern = false;      // fast path: reflected Halpern PDHG on the FP64-recentred problem
     double rho = 1.9;          // its reflection coefficient
@@ -67,14 +66,12 @@ struct Outcome {
 Outcome run_pdhg(Device& dev, const Scaled& P, const Options& o, double time_budget, const double* x0 = nullptr,
                  const double* z0 = nullptr, double omega0 = 0, double norm_a = 0);
 
-// The fast path (Options::halpern): reflected Halpern PDHG iterations on the FastDevice with the same restart
-// tests and primal-weight update as run_pdhg, FP64 evaluation at base + inner point, and, when o.refine, FP64
-// iterative refinement: the first round ends when its relative KKT has dropped by inner_tol or when it stalls (no
-// halving for as many iterations as its last halving took, at least 8192) — FP32 products put a floor under the
-// first round's evaluated gap; later rounds end only on a longer stall (twice that, at least 32768); the base moves to the round's point and the next round
-// solves the recentred problem with exact FP64 base products, keeping the primal weight. A point that meets the
-// tolerance is confirmed by an FP64 evaluation at base + inner before the solve stops. x0/z0 (scaled
-// space, nullable) become the first base point (warm start).
+// The fast path (Options::halpern): reflected Halpern PDHG iterations on the FastDevice with the restart tests and
+// primal-weight update of run_pdhg (the candidate is the current point). With o.refine (FP64 iterative
+// refinement) every restart also recentres: the base point moves to the restart point in FP64 and the next inner
+// problem is rebuilt with exact FP64 products, so the FP32 inner iterates only ever span one restart cycle (at most
+// o.max_refinements recentrings). A point that meets the tolerance is confirmed by an FP64 evaluation after a
+// recentring before the solve stops. x0/z0 (scaled space, nullable) become the first base point (warm start).
 Outcome run_fast(FastDevice& dev, const Scaled& P, const Options& o, double time_budget, const double* x0 = nullptr,
                  const double* z0 = nullptr, double omega0 = 0, double norm_a = 0);
 
