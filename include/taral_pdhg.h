@@ -218,6 +218,35 @@ void tpdhg_session_free(tpdhg_session* s);
 int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const tpdhg_result* result, char* buf,
                       int32_t len);
 
+/* ------------------------------------------------------------------ batches (no reference counterpart) */
+/* Many LPs solved together by the fast path on one device: block-diagonal packing, one row launch and one column
+ * launch per iteration for all live instances, per-instance step sizes, restarts and stopping, one HIP graph per
+ * evaluation block. Instances whose scaled matrices are identical (scenario families) share one device copy.
+ * Each instance follows exactly the decisions a single tpdhg_solve with the same options would take. */
+typedef struct tpdhg_batch tpdhg_batch;
+typedef struct {
+    int32_t count, unique_matrices;
+    int32_t cap;              /* nonzeros staged per row block (1024, 2048 or 4096) */
+    int64_t nnz_total;        /* over all instances */
+    int64_t nnz_stored;       /* held on the device (shared matrices once) */
+    int64_t device_mem_bytes;
+    double setup_s;           /* scaling, upload and stage-size autotuning */
+    float tune_ms[3];         /* autotune timings for 1024 / 2048 / 4096 (0 when cap was given) */
+} tpdhg_batch_info;
+/* opt->halpern must be 1. cap: 0 = pick the row-block stage by timing, else 1024, 2048 or 4096. The batch keeps
+ * copies of the models. */
+int tpdhg_batch_create(const tpdhg_model* const* models, int32_t count, const tpdhg_options* opt, int32_t cap,
+                       tpdhg_batch** out);
+int tpdhg_batch_get_info(const tpdhg_batch* b, tpdhg_batch_info* info);
+/* New cost and bounds (original space) for instance i; its matrix and scaling stay. */
+int tpdhg_batch_set_vectors(tpdhg_batch* b, int32_t i, const double* cost, const double* col_lo, const double* col_up,
+                            const double* row_lo, const double* row_up);
+/* Solves every instance. x0/y0/primal_weight: NULL, or arrays of count entries (NULL entries: cold start);
+ * results: count entries; x/y: NULL or count pointers (NULL entries skip that copy). */
+int tpdhg_batch_solve(tpdhg_batch* b, const double* const* x0, const double* const* y0, const double* primal_weight,
+                      tpdhg_result* results, double* const* x, double* const* y);
+void tpdhg_batch_free(tpdhg_batch* b);
+
 /* ------------------------------------------------------------------ learned warm start (no reference counterpart) */
 /* MLP n_in -> h1 -> h2 -> n_out with ReLU, bf16 MFMA matrix products with FP32 accumulation, FP32 master
  * weights, Adam training on the GPU. n_in, h1, h2: positive multiples of 16. Parameters (FP32, row-major):

@@ -1,0 +1,892 @@
+// Batched fast-path backend (batch.hpp): block-diagonal packing of many LPs.
+//
+// Layout in HBM. Row vectors of all instances are concatenated (instance i owns rows [roff_i, roff_i + m_i)),
+// column vectors likewise ([coff_i, coff_i + n_i)). Matrices live in pools: one CSR (row pointers holding pool
+// offsets, local column indices, values in the iterate type and in FP64) per distinct scaled matrix, for A and
+// for A^T; instance i reads its rows through pbase_i (its first row pointer in the pool) and gathers from its own
+// segment of the vector. A row-block table lists (instance, first row, end row) for every instance's row blocks,
+// so one launch covers the whole batch; a workgroup whose instance is idle returns at once. Reductions use a
+// second table with red_blocks(size_i) entries per instance, the single-instance partition, so sums come out
+// bit-identical to the FastDevice's.
+//
+// Per iteration the batch costs one row launch and one column launch; a block of eval_freq iterations is one HIP
+// graph whose kernels read the activity flags and the per-instance step sizes from device memory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+
+#include "batch.hpp"
+#include "hipcommon.hpp"
+
+namespace tpdhg {
+namespace {
+
+template <class T>
+__device__ __forceinline__ T batch_lambda(int k) {
+    return T(double(k + 1) / double(k + 2));
+}
+
+// y[off_i + r] = s
+template <class T>
+struct BStore {
+    T* y;
+    const int* off;
+    int base;
+    __device__ void load(int i) { base = off[i]; }
+    __device__ void operator()(int r, T s) const { y[base + r] = s; }
+};
+
+// Dual prox step and Halpern blend of the fast path (fast.hip EpiHRow), per instance scalars.
+template <class T>
+struct BEpiHRow {
+    T* z;
+    const T *z0, *zh, *rl, *ru, *par;
+    const int *kc, *off;
+    int s;
+    T rho;
+    int base;
+    T sig, lam;
+    __device__ void load(int i) { base = off[i], sig = par[2 * i + 1], lam = batch_lambda<T>(kc[i] + s); }
+    __device__ void operator()(int r, T a) const {
+        const int g = base + r;
+        const T zi = z[g], h = zh[g];
+        const T v = zi + sig * a, q = (v + h) / sig;
+        const T lo = rl[g], hi = ru[g];
+        const T zp = q < lo ? v - sig * lo : (q > hi ? v - sig * hi : -h);
+        z[g] = lam * (rho * zp - (rho - 1) * zi) + (1 - lam) * z0[g];
+    }
+};
+
+// Primal Halpern blend (mode 0) or none (mode 1), next primal step and gather vector (fast.hip EpiHCol).
+template <class T>
+struct BEpiHCol {
+    T *x, *xp, *w, *ATz;
+    const T *x0, *c, *l, *u, *par;
+    const int *kc, *off;
+    int s, mode;
+    T rho;
+    int base;
+    T tau, lam;
+    __device__ void load(int i) { base = off[i], tau = par[2 * i], lam = batch_lambda<T>(kc[i] + s); }
+    __device__ void operator()(int r, T a) const {
+        const int j = base + r;
+        T xj = x[j];
+        if (mode == 0) {
+            xj = lam * (rho * xp[j] - (rho - 1) * xj) + (1 - lam) * x0[j];
+            x[j] = xj;
+        }
+        const T xn = clampv(xj - tau * (c[j] + a), l[j], u[j]);
+        xp[j] = xn, w[j] = 2 * xn - xj;
+        if (ATz) ATz[j] = a;
+    }
+};
+
+// Row-block SpMV over the batch (the k_rowblocks scheme of hipcommon.hpp with a CAP-entry stage). blk holds
+// (instance, first row, end row) triples; rows are local to the instance.
+template <class T, int CAP, class Epi>
+__global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict__ blk, const int* __restrict__ pbase,
+                                                 const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                 const T* __restrict__ val, const int* __restrict__ voff,
+                                                 const T* __restrict__ v, const uint8_t* __restrict__ on, Epi epi) {
+    __shared__ T sh[CAP];
+    __shared__ T red[kWG / 64];
+    const int b = xcd_block(int(blockIdx.x), nblk);
+    const int inst = blk[3 * b], r0 = blk[3 * b + 1], r1 = blk[3 * b + 2];
+    if (!on[inst]) return;  // the whole workgroup leaves together
+    const int* P = ptr + pbase[inst];
+    const T* vi = v + voff[inst];
+    const int tid = int(threadIdx.x);
+    const int k0 = P[r0], k1 = P[r1];
+    epi.load(inst);
+    if (k1 - k0 > CAP) {
+        T s = 0;
+        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * vi[idx[k]];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) epi(r0, (red[0] + red[1]) + (red[2] + red[3]));
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < CAP / kWG; ++u) {
+        const int k = k0 + u * kWG + tid;
+        if (k < k1) sh[k - k0] = val[k] * vi[idx[k]];
+    }
+    __syncthreads();
+    const int nr = r1 - r0;
+    int lg = 6;
+    while (lg > 0 && (nr << lg) > kWG) --lg;
+    const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
+    T s = 0;
+    if (row < nr) {
+        const int a = P[r0 + row] - k0, e = P[r0 + row + 1] - k0;
+        for (int k = a + lane; k < e; k += lpr) s += sh[k];
+    }
+    for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (row < nr && lane == 0) epi(r0 + row, s);
+}
+
+// Reduction / elementwise table: (instance, q, nb) entries; entry q of nb covers local indices
+// q * kWG + t, stepping nb * kWG (the single-instance grid-stride partition with nb = red_blocks(size)).
+struct Seg {
+    const int* tab;
+    const int* off;   // first global index of each instance
+    const int* size;  // element count of each instance
+};
+
+#define SEG_LOOP(S, on, ...)                                                          \
+    const int inst_ = (S).tab[3 * blockIdx.x], q_ = (S).tab[3 * blockIdx.x + 1];      \
+    const int nb_ = (S).tab[3 * blockIdx.x + 2];                                       \
+    const bool act_ = (on)[inst_] != 0;                                                \
+    const int base_ = (S).off[inst_], size_ = (S).size[inst_];                         \
+    if (act_)                                                                          \
+        for (int r_ = q_ * kWG + int(threadIdx.x); r_ < size_; r_ += nb_ * kWG) {      \
+            const int g = base_ + r_;                                                  \
+            __VA_ARGS__                                                                \
+        }
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_bbegin_rows(Seg S, const uint8_t* __restrict__ on, const double* __restrict__ rl,
+                                                     const double* __restrict__ ru, const double* __restrict__ Axb,
+                                                     const double* __restrict__ zb, T* __restrict__ rl_in,
+                                                     T* __restrict__ ru_in, T* __restrict__ zh, T* __restrict__ z,
+                                                     T* __restrict__ z0) {
+    SEG_LOOP(S, on, {
+        rl_in[g] = T(rl[g] - Axb[g]), ru_in[g] = T(ru[g] - Axb[g]), zh[g] = T(zb[g]);
+        z[g] = 0, z0[g] = 0;
+    })
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_bbegin_cols(Seg S, const uint8_t* __restrict__ on, const double* __restrict__ c,
+                                                     const double* __restrict__ l, const double* __restrict__ u,
+                                                     const double* __restrict__ ATzb, const double* __restrict__ xb,
+                                                     T* __restrict__ c_in, T* __restrict__ l_in, T* __restrict__ u_in,
+                                                     T* __restrict__ x, T* __restrict__ x0) {
+    SEG_LOOP(S, on, {
+        const T lo = T(l[g] - xb[g]), hi = T(u[g] - xb[g]);
+        c_in[g] = T(c[g] + ATzb[g]), l_in[g] = lo, u_in[g] = hi;
+        const T v = clampv(T(0), lo, hi);
+        x[g] = v, x0[g] = v;
+    })
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_baccumulate(Seg S, const uint8_t* __restrict__ on, double* __restrict__ base,
+                                                     const T* __restrict__ inner) {
+    SEG_LOOP(S, on, { base[g] += double(inner[g]); })
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_bcopy(Seg S, const uint8_t* __restrict__ on, T* __restrict__ dst,
+                                               const T* __restrict__ src) {
+    SEG_LOOP(S, on, { dst[g] = src[g]; })
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_bscale(Seg S, const uint8_t* __restrict__ on, T* __restrict__ x,
+                                                const double* __restrict__ fac) {
+    SEG_LOOP(S, on, { x[g] = T(x[g] * fac[inst_]); })
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_bsumsq(Seg S, const uint8_t* __restrict__ on, const T* __restrict__ x,
+                                                double* __restrict__ part) {
+    double v[1] = {0};
+    SEG_LOOP(S, on, { v[0] += double(x[g]) * double(x[g]); })
+    block_store<1>(v, part);
+}
+
+__global__ void k_bset_counters(int B, const uint8_t* __restrict__ on, int* __restrict__ kc, int add, int reset) {
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < B; i += int(gridDim.x * blockDim.x))
+        if (on[i]) kc[i] = reset ? 0 : kc[i] + add;
+}
+
+template <class T>
+struct BEvalArgs {
+    const T *Ax, *z, *z0, *x, *ATz, *x0;
+    const double *Axb, *zb, *xb, *ATzb;
+    const double *rl, *ru, *Dr, *c, *l, *u, *Dc;
+};
+
+// KKT slots of base + inner per instance (fast.hip k_eval_fast, per segment).
+template <class T>
+__global__ __launch_bounds__(kWG) void k_beval_rows(Seg S, const uint8_t* __restrict__ on, BEvalArgs<T> a,
+                                                    double* __restrict__ part) {
+    double v[5] = {0, 0, 0, 0, 0};
+    SEG_LOOP(S, on, {
+        const double ax = a.Axb[g] + double(a.Ax[g]), zz = a.zb[g] + double(a.z[g]);
+        const double lo = a.rl[g], up = a.ru[g], dr = a.Dr[g];
+        const double r = ax < lo ? lo - ax : (ax > up ? ax - up : 0), ro = r / dr;
+        const double rr = ro * row_rel_weight(lo, up, dr);
+        v[0] += r * r, v[1] += ro * ro, v[2] += rr * rr;
+        if (zz > 0 && up < INFINITY) v[3] -= zz * up;
+        if (zz < 0 && lo > -INFINITY) v[3] -= zz * lo;
+        const double dz = double(a.z[g]) - double(a.z0[g]);
+        v[4] += dz * dz;
+    })
+    block_store<5>(v, part);
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_beval_cols(Seg S, const uint8_t* __restrict__ on, BEvalArgs<T> a,
+                                                    double* __restrict__ part) {
+    double v[5] = {0, 0, 0, 0, 0};
+    SEG_LOOP(S, on, {
+        const double xx = a.xb[g] + double(a.x[g]), cj = a.c[g];
+        const double d = cj + a.ATzb[g] + double(a.ATz[g]), lo = a.l[g], up = a.u[g];
+        double res = 0;
+        v[2] += cj * xx;
+        if (d > 0) {
+            if (lo > -INFINITY) v[3] += d * lo;
+            else res = d;
+        } else if (d < 0) {
+            if (up < INFINITY) v[3] += d * up;
+            else res = -d;
+        }
+        const double ro = res / a.Dc[g], dx = double(a.x[g]) - double(a.x0[g]);
+        v[0] += res * res, v[1] += ro * ro, v[4] += dx * dx;
+    })
+    block_store<5>(v, part);
+}
+
+// Per instance: out[inst * stride + ooff + k] = sum of its K partials (k_finish per segment, same order).
+__global__ __launch_bounds__(kWG) void k_bfinish(const int* __restrict__ first, const int* __restrict__ count,
+                                                 const uint8_t* __restrict__ on, int K, const double* __restrict__ part,
+                                                 int stride, int ooff, double* __restrict__ out) {
+    __shared__ double sh[kWG];
+    const int inst = int(blockIdx.x), t = int(threadIdx.x);
+    if (!on[inst]) return;
+    const int b0 = first[inst], blocks = count[inst];
+    for (int k = 0; k < K; ++k) {
+        double s = 0;
+        for (int q = t; q < blocks; q += kWG) s += part[size_t(b0 + q) * size_t(K) + size_t(k)];
+        sh[t] = s;
+        __syncthreads();
+        for (int o = kWG / 2; o > 0; o >>= 1) {
+            if (t < o) sh[t] += sh[t + o];
+            __syncthreads();
+        }
+        if (t == 0) out[size_t(inst) * size_t(stride) + size_t(ooff + k)] = sh[0];
+        __syncthreads();
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kWG) void k_bcombine(int k, const double* __restrict__ base, const T* __restrict__ inner,
+                                                  double* __restrict__ out) {
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < k; i += int(gridDim.x * blockDim.x))
+        out[i] = base[i] + double(inner[i]);
+}
+
+// Row blocks with a stage of `cap` nonzeros (row_blocks of hipcommon.hpp for cap = kCap).
+std::vector<int32_t> row_blocks_cap(int rows, const int32_t* ptr, int cap) {
+    std::vector<int32_t> b;
+    b.push_back(0);
+    int r0 = 0;
+    while (r0 < rows) {
+        int r = r0;
+        while (r < rows && r - r0 < kWG && int64_t(ptr[r + 1]) - ptr[r0] <= cap) ++r;
+        if (r == r0) r = r0 + 1;
+        b.push_back(r);
+        r0 = r;
+    }
+    return b;
+}
+
+struct Pool {  // distinct matrices of one side, host copy
+    std::vector<int32_t> ptr, idx;
+    std::vector<double> val;
+    std::vector<int32_t> pbase;   // per distinct matrix: first row pointer in ptr
+    std::vector<int32_t> rows;    // per distinct matrix
+};
+
+uint64_t mix_hash(uint64_t h, uint64_t v) {
+    h ^= v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+    return h;
+}
+
+uint64_t csr_hash(const Csr& a) {
+    uint64_t h = mix_hash(uint64_t(a.rows), uint64_t(a.nnz()));
+    const size_t step = std::max<size_t>(1, a.val.size() / 4096);
+    for (size_t k = 0; k < a.val.size(); k += step) {
+        uint64_t bits;
+        std::memcpy(&bits, &a.val[k], sizeof bits);
+        h = mix_hash(h, bits ^ (uint64_t(uint32_t(a.idx[k])) << 32));
+    }
+    return h;
+}
+
+bool csr_equal(const Csr& a, const Csr& b) {
+    return a.rows == b.rows && a.ptr == b.ptr && a.idx == b.idx && a.val == b.val;
+}
+
+template <class T>
+class BatchDev final : public BatchDevice {
+public:
+    BatchDev(const std::vector<const Scaled*>& probs, const BatchOptions& o) : B_(int(probs.size())), o_(o) {
+        require_device();
+        if (B_ <= 0) throw std::runtime_error("empty batch");
+        if (!(o.rho >= 1 && o.rho < 2)) throw std::runtime_error("reflection coefficient rho must be in [1, 2)");
+        if (o.eval_freq < 1) throw std::runtime_error("eval_freq must be >= 1");
+        if (o.cap != 0 && o.cap != 1024 && o.cap != 2048 && o.cap != 4096)
+            throw std::runtime_error("batch row-block stage must be 0 (autotune), 1024, 2048 or 4096 nonzeros");
+        size_t free0 = 0, free1 = 0, total = 0;
+        HIP_OK(hipMemGetInfo(&free0, &total));
+        name_ = device_name();
+        HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        build_host(probs);
+        upload_all(probs);
+        HIP_OK(hipMemGetInfo(&free1, &total));
+        mem_used_ = free0 > free1 ? free0 - free1 : 0;
+        if (o.cap) {
+            set_cap(o.cap);
+        } else {
+            autotune();
+        }
+    }
+    ~BatchDev() override {
+        if (graph_) (void)hipGraphExecDestroy(graph_);
+        for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+        if (hflags_) (void)hipHostFree(hflags_);
+        if (hpar_) (void)hipHostFree(hpar_);
+        if (hout_) (void)hipHostFree(hout_);
+        if (st_) (void)hipStreamDestroy(st_);
+    }
+
+    int count() const override { return B_; }
+    int m(int i) const override { return m_[size_t(i)]; }
+    int n(int i) const override { return n_[size_t(i)]; }
+    int64_t nnz_stored() const override { return int64_t(PA_.val.size()); }
+    int64_t nnz_total() const override { return nnz_total_; }
+    int unique_matrices() const override { return int(PA_.rows.size()); }
+    int cap() const override { return cap_; }
+    std::vector<float> tune_ms() const override { return tune_ms_; }
+
+    std::vector<double> power_norm(int iters) override {
+        std::vector<double> est(static_cast<size_t>(B_), 0.0), prev(static_cast<size_t>(B_), 0.0), fac(static_cast<size_t>(B_), 1.0);
+        {
+            std::vector<T> hv(static_cast<size_t>(ntot_));
+            for (int i = 0; i < B_; ++i) {
+                double nn = 0;
+                std::vector<double> h(static_cast<size_t>(n_[size_t(i)]));
+                for (int j = 0; j < n_[size_t(i)]; ++j) {
+                    h[size_t(j)] = 0.5 + double((uint32_t(j) * 2654435761u) % 1000u) / 1000.0;
+                    nn += h[size_t(j)] * h[size_t(j)];
+                }
+                for (int j = 0; j < n_[size_t(i)]; ++j)
+                    hv[size_t(coff_[size_t(i)]) + size_t(j)] = T(h[size_t(j)] / std::sqrt(nn));
+            }
+            HIP_OK(hipMemcpyAsync(pv_, hv.data(), hv.size() * sizeof(T), hipMemcpyHostToDevice, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+        }
+        std::vector<uint8_t> on(static_cast<size_t>(B_), 1);
+        T *pv = pv_, *pu = pu_;
+        for (int it = 0; it < iters; ++it) {
+            upload_flags(on);
+            spmv_rows(A_, pv, BStore<T>{pw_, d_roff_, 0}, dflags_);
+            spmv_rows(AT_, pw_, BStore<T>{pu, d_coff_, 0}, dflags_);
+            k_bsumsq<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, pu, part_);
+            k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst_, d_ccount_, dflags_, 1, part_, 1, 0, out_);
+            HIP_OK(hipMemcpyAsync(hout_, out_, size_t(B_) * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+            bool any = false;
+            for (int i = 0; i < B_; ++i) {
+                if (!on[size_t(i)]) continue;
+                const double s = hout_[i];
+                prev[size_t(i)] = est[size_t(i)];
+                if (s <= 0) {
+                    est[size_t(i)] = 0, on[size_t(i)] = 0;
+                    continue;
+                }
+                est[size_t(i)] = std::sqrt(std::sqrt(s));
+                fac[size_t(i)] = 1 / std::sqrt(s);
+            }
+            HIP_OK(hipMemcpyAsync(fac_, fac.data(), size_t(B_) * sizeof(double), hipMemcpyHostToDevice, st_));
+            upload_flags(on);
+            k_bscale<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, pu, fac_);
+            HIP_OK(hipStreamSynchronize(st_));
+            std::swap(pv, pu);
+            for (int i = 0; i < B_; ++i) {
+                if (!on[size_t(i)]) continue;
+                if (it >= 10 && std::fabs(est[size_t(i)] - prev[size_t(i)]) <= 1e-7 * est[size_t(i)]) on[size_t(i)] = 0;
+                any = any || on[size_t(i)];
+            }
+            if (!any) break;
+        }
+        HIP_OK(hipGetLastError());
+        return est;
+    }
+
+    void set_base(const std::vector<const double*>& xb, const std::vector<const double*>& zb) override {
+        std::vector<double> hx(static_cast<size_t>(ntot_), 0.0), hz(static_cast<size_t>(mtot_), 0.0);
+        for (int i = 0; i < B_; ++i) {
+            if (size_t(i) < xb.size() && xb[size_t(i)])
+                std::copy(xb[size_t(i)], xb[size_t(i)] + n_[size_t(i)], hx.begin() + coff_[size_t(i)]);
+            if (size_t(i) < zb.size() && zb[size_t(i)])
+                std::copy(zb[size_t(i)], zb[size_t(i)] + m_[size_t(i)], hz.begin() + roff_[size_t(i)]);
+        }
+        HIP_OK(hipMemcpyAsync(xb_, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipMemcpyAsync(zb_, hz.data(), hz.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+
+    void begin_round(const std::vector<uint8_t>& on) override {
+        upload_flags(on);
+        spmv_rows(A64_, xb_, BStore<double>{Axb_, d_roff_, 0}, dflags_);
+        spmv_rows(AT64_, zb_, BStore<double>{ATzb_, d_coff_, 0}, dflags_);
+        k_bbegin_rows<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, rl64_, ru64_, Axb_, zb_, rl_in_, ru_in_, zh_, z_, z0_);
+        k_bbegin_cols<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, c64_, l64_, u64_, ATzb_, xb_, c_in_, l_in_, u_in_,
+                                                  x_, x0_);
+        HIP_OK(hipStreamSynchronize(st_));
+        HIP_OK(hipGetLastError());
+    }
+
+    void set_steps(const std::vector<double>& tau, const std::vector<double>& sigma) override {
+        for (int i = 0; i < B_; ++i) hpar_[2 * i] = T(tau[size_t(i)]), hpar_[2 * i + 1] = T(sigma[size_t(i)]);
+        HIP_OK(hipMemcpyAsync(par_, hpar_, size_t(2 * B_) * sizeof(T), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+
+    void restart(const std::vector<uint8_t>& on) override {
+        upload_flags(on);
+        k_bcopy<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, x0_, x_);
+        k_bcopy<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, z0_, z_);
+        k_bset_counters<<<(B_ + kWG - 1) / kWG, kWG, 0, st_>>>(B_, dflags_, kc_, 0, 1);
+        spmv_rows(AT_, z_, col_epi(0, 1, true), dflags_);
+        HIP_OK(hipStreamSynchronize(st_));
+        HIP_OK(hipGetLastError());
+    }
+
+    void end_round(const std::vector<uint8_t>& on) override {
+        upload_flags(on);
+        k_baccumulate<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, xb_, x_);
+        k_baccumulate<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, zb_, z_);
+        HIP_OK(hipStreamSynchronize(st_));
+        HIP_OK(hipGetLastError());
+    }
+
+    void iterate(int steps, const std::vector<uint8_t>& on) override {
+        if (steps <= 0) return;
+        upload_act(on);
+        if (o_.use_graph && !profile_ && steps == o_.eval_freq) {
+            if (!graph_) {
+                hipGraph_t g = nullptr;
+                HIP_OK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+                try {
+                    launch_block(steps);
+                } catch (...) {
+                    (void)hipStreamEndCapture(st_, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                HIP_OK(hipStreamEndCapture(st_, &g));
+                HIP_OK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+                HIP_OK(hipGraphDestroy(g));
+            }
+            HIP_OK(hipGraphLaunch(graph_, st_));
+        } else {
+            launch_block(steps);
+        }
+        HIP_OK(hipGetLastError());
+    }
+
+    void eval(const std::vector<uint8_t>& on, std::vector<double>& out) override {
+        upload_flags(on);
+        spmv_rows(A_, x_, BStore<T>{Ax_, d_roff_, 0}, dflags_);
+        BEvalArgs<T> a{Ax_, z_, z0_, x_, ATz_, x0_, Axb_, zb_, xb_, ATzb_, rl64_, ru64_, Dr64_, c64_, l64_, u64_, Dc64_};
+        k_beval_rows<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, a, part_);
+        k_beval_cols<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, a, part_ + size_t(nrseg_) * 5);
+        k_bfinish<<<B_, kWG, 0, st_>>>(d_rfirst_, d_rcount_, dflags_, 5, part_, NKKT, 0, out_);
+        k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst5_, d_ccount_, dflags_, 5, part_, NKKT, 5, out_);
+        HIP_OK(hipMemcpyAsync(hout_, out_, size_t(B_) * NKKT * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        HIP_OK(hipGetLastError());
+        out.resize(size_t(B_) * NKKT);
+        for (int i = 0; i < B_; ++i)
+            if (on[size_t(i)]) std::copy(hout_ + size_t(i) * NKKT, hout_ + size_t(i + 1) * NKKT, out.begin() + size_t(i) * NKKT);
+        collect_events();
+    }
+
+    void download(int i, double* hx, double* hz) override {
+        const int n = n_[size_t(i)], m = m_[size_t(i)];
+        if (hx && n) {
+            k_bcombine<T><<<grid_for(n), kWG, 0, st_>>>(n, xb_ + coff_[size_t(i)], x_ + coff_[size_t(i)], buf64_);
+            HIP_OK(hipMemcpyAsync(hx, buf64_, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+        }
+        if (hz && m) {
+            k_bcombine<T><<<grid_for(m), kWG, 0, st_>>>(m, zb_ + roff_[size_t(i)], z_ + roff_[size_t(i)], buf64_);
+            HIP_OK(hipMemcpyAsync(hz, buf64_, size_t(m) * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+        }
+        HIP_OK(hipGetLastError());
+    }
+
+    void set_vectors(int i, const double* c, const double* l, const double* u, const double* rl,
+                     const double* ru) override {
+        if (i < 0 || i >= B_) throw std::invalid_argument("instance index out of range");
+        const size_t n = size_t(n_[size_t(i)]), m = size_t(m_[size_t(i)]);
+        const size_t co = size_t(coff_[size_t(i)]), ro = size_t(roff_[size_t(i)]);
+        auto put = [&](double* dst, const double* src, size_t k) {
+            if (k) HIP_OK(hipMemcpyAsync(dst, src, k * sizeof(double), hipMemcpyHostToDevice, st_));
+        };
+        put(c64_ + co, c, n), put(l64_ + co, l, n), put(u64_ + co, u, n), put(rl64_ + ro, rl, m), put(ru64_ + ro, ru, m);
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+
+    void set_profile(bool on) override { profile_ = on; }
+    KernelTimes kernel_times() override {
+        HIP_OK(hipStreamSynchronize(st_));
+        collect_events();
+        KernelTimes t = times_;
+        times_ = KernelTimes();
+        return t;
+    }
+    const std::string& name() const override { return name_; }
+    size_t mem_used() const override { return mem_used_; }
+
+private:
+    struct Side {  // device copy of one side's pool and the block table for the current stage size
+        int rows_total = 0, nblk = 0;
+        int *ptr = nullptr, *idx = nullptr, *pbase = nullptr, *voff = nullptr, *blk = nullptr;
+        T* val = nullptr;
+    };
+    struct Side64 {
+        int nblk = 0;
+        int *ptr = nullptr, *idx = nullptr, *pbase = nullptr, *voff = nullptr, *blk = nullptr;
+        double* val = nullptr;
+    };
+
+    void build_host(const std::vector<const Scaled*>& probs) {
+        m_.resize(size_t(B_)), n_.resize(size_t(B_)), roff_.resize(size_t(B_)), coff_.resize(size_t(B_));
+        mat_of_.resize(size_t(B_));
+        std::multimap<uint64_t, int> seen;  // hash -> distinct matrix index
+        std::vector<const Scaled*> uniq;
+        int64_t mt = 0, nt = 0;
+        for (int i = 0; i < B_; ++i) {
+            const Scaled& P = *probs[size_t(i)];
+            if (P.m <= 0 || P.n <= 0) throw std::runtime_error("batch instances need at least one row and one column");
+            m_[size_t(i)] = P.m, n_[size_t(i)] = P.n, roff_[size_t(i)] = int(mt), coff_[size_t(i)] = int(nt);
+            mt += P.m, nt += P.n;
+            nnz_total_ += P.A.nnz();
+            const uint64_t h = csr_hash(P.A);
+            int found = -1;
+            for (auto it = seen.lower_bound(h); it != seen.end() && it->first == h; ++it)
+                if (csr_equal(uniq[size_t(it->second)]->A, P.A)) {
+                    found = it->second;
+                    break;
+                }
+            if (found < 0) {
+                found = int(uniq.size());
+                uniq.push_back(&P);
+                seen.emplace(h, found);
+            }
+            mat_of_[size_t(i)] = found;
+        }
+        if (mt > INT32_MAX || nt > INT32_MAX) throw std::runtime_error("batch too large for 32-bit vector offsets");
+        mtot_ = int(mt), ntot_ = int(nt);
+        auto add = [](Pool& pool, const Csr& a) {
+            const int64_t at = int64_t(pool.val.size());
+            if (at + a.nnz() > INT32_MAX) throw std::runtime_error("batch matrices exceed 2^31 stored nonzeros");
+            pool.pbase.push_back(int32_t(pool.ptr.size()));
+            pool.rows.push_back(a.rows);
+            for (int32_t p : a.ptr) pool.ptr.push_back(int32_t(at + p));
+            pool.idx.insert(pool.idx.end(), a.idx.begin(), a.idx.end());
+            pool.val.insert(pool.val.end(), a.val.begin(), a.val.end());
+        };
+        for (const Scaled* P : uniq) add(PA_, P->A), add(PT_, P->AT);
+        // reduction / elementwise tables (single-instance partition)
+        auto seg = [&](const std::vector<int>& size, std::vector<int32_t>& tab, std::vector<int32_t>& first,
+                       std::vector<int32_t>& count) {
+            for (int i = 0; i < B_; ++i) {
+                const int nb = red_blocks(size[size_t(i)]);
+                first.push_back(int32_t(tab.size() / 3)), count.push_back(nb);
+                for (int q = 0; q < nb; ++q) tab.push_back(i), tab.push_back(q), tab.push_back(nb);
+            }
+        };
+        seg(m_, rtab_, rfirst_, rcount_);
+        seg(n_, ctab_, cfirst_, ccount_);
+        nrseg_ = int(rfirst_.empty() ? 0 : rtab_.size() / 3), ncseg_ = int(ctab_.size() / 3);
+    }
+
+    int* up_int(const std::vector<int32_t>& v) {
+        int* p = mem_.alloc<int>(std::max<size_t>(v.size(), 1));
+        if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        return p;
+    }
+    template <class U>
+    U* up_vals(const std::vector<double>& v) {
+        std::vector<U> h(v.size());
+        for (size_t k = 0; k < v.size(); ++k) h[k] = U(v[k]);
+        U* p = mem_.alloc<U>(std::max<size_t>(v.size(), 1));
+        if (!h.empty()) HIP_OK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        return p;
+    }
+    double* cat64(const std::vector<const Scaled*>& probs, int rowside, std::vector<double> Scaled::*f) {
+        std::vector<double> h(static_cast<size_t>(rowside ? mtot_ : ntot_));
+        for (int i = 0; i < B_; ++i) {
+            const std::vector<double>& v = probs[size_t(i)]->*f;
+            std::copy(v.begin(), v.end(), h.begin() + (rowside ? roff_[size_t(i)] : coff_[size_t(i)]));
+        }
+        double* p = mem_.alloc<double>(std::max<size_t>(h.size(), 1));
+        HIP_OK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        return p;
+    }
+    T* zerosT(int k) {
+        T* p = mem_.alloc<T>(size_t(std::max(k, 1)));
+        HIP_OK(hipMemsetAsync(p, 0, size_t(std::max(k, 1)) * sizeof(T), st_));
+        return p;
+    }
+    double* zeros64(int k) {
+        double* p = mem_.alloc<double>(size_t(std::max(k, 1)));
+        HIP_OK(hipMemsetAsync(p, 0, size_t(std::max(k, 1)) * sizeof(double), st_));
+        return p;
+    }
+
+    void upload_all(const std::vector<const Scaled*>& probs) {
+        // matrices: pointers and indices once; values in the iterate type and in FP64 (shared when T is double)
+        std::vector<int32_t> pa(static_cast<size_t>(B_)), pt(static_cast<size_t>(B_));
+        for (int i = 0; i < B_; ++i)
+            pa[size_t(i)] = PA_.pbase[size_t(mat_of_[size_t(i)])], pt[size_t(i)] = PT_.pbase[size_t(mat_of_[size_t(i)])];
+        A_.ptr = up_int(PA_.ptr), A_.idx = up_int(PA_.idx), A_.pbase = up_int(pa), A_.voff = up_int(coff_);
+        AT_.ptr = up_int(PT_.ptr), AT_.idx = up_int(PT_.idx), AT_.pbase = up_int(pt), AT_.voff = up_int(roff_);
+        A_.rows_total = mtot_, AT_.rows_total = ntot_;
+        A_.val = up_vals<T>(PA_.val), AT_.val = up_vals<T>(PT_.val);
+        A64_.ptr = A_.ptr, A64_.idx = A_.idx, A64_.pbase = A_.pbase, A64_.voff = A_.voff;
+        AT64_.ptr = AT_.ptr, AT64_.idx = AT_.idx, AT64_.pbase = AT_.pbase, AT64_.voff = AT_.voff;
+        if constexpr (sizeof(T) == sizeof(double)) {
+            A64_.val = reinterpret_cast<double*>(A_.val), AT64_.val = reinterpret_cast<double*>(AT_.val);
+        } else {
+            A64_.val = up_vals<double>(PA_.val), AT64_.val = up_vals<double>(PT_.val);
+        }
+        d_roff_ = up_int(roff_), d_coff_ = up_int(coff_);
+        d_m_ = up_int(m_), d_n_ = up_int(n_);
+        d_rtab_ = up_int(rtab_), d_ctab_ = up_int(ctab_);
+        d_rfirst_ = up_int(rfirst_), d_rcount_ = up_int(rcount_), d_cfirst_ = up_int(cfirst_), d_ccount_ = up_int(ccount_);
+        {
+            std::vector<int32_t> c5(cfirst_);
+            for (int32_t& v : c5) v += nrseg_;  // column partials follow the row partials in part_
+            d_cfirst5_ = up_int(c5);
+        }
+        c64_ = cat64(probs, 0, &Scaled::c), l64_ = cat64(probs, 0, &Scaled::l), u64_ = cat64(probs, 0, &Scaled::u);
+        Dc64_ = cat64(probs, 0, &Scaled::Dc);
+        rl64_ = cat64(probs, 1, &Scaled::rl), ru64_ = cat64(probs, 1, &Scaled::ru), Dr64_ = cat64(probs, 1, &Scaled::Dr);
+        xb_ = zeros64(ntot_), zb_ = zeros64(mtot_), Axb_ = zeros64(mtot_), ATzb_ = zeros64(ntot_);
+        int maxlen = 1;
+        for (int i = 0; i < B_; ++i) maxlen = std::max(maxlen, std::max(m_[size_t(i)], n_[size_t(i)]));
+        buf64_ = zeros64(maxlen);
+        c_in_ = zerosT(ntot_), l_in_ = zerosT(ntot_), u_in_ = zerosT(ntot_);
+        rl_in_ = zerosT(mtot_), ru_in_ = zerosT(mtot_), zh_ = zerosT(mtot_);
+        x_ = zerosT(ntot_), xp_ = zerosT(ntot_), w_ = zerosT(ntot_), x0_ = zerosT(ntot_), ATz_ = zerosT(ntot_);
+        z_ = zerosT(mtot_), z0_ = zerosT(mtot_), Ax_ = zerosT(mtot_);
+        pv_ = zerosT(ntot_), pw_ = zerosT(mtot_), pu_ = zerosT(ntot_);
+        par_ = zerosT(2 * B_);
+        kc_ = mem_.alloc<int>(size_t(B_));
+        HIP_OK(hipMemsetAsync(kc_, 0, size_t(B_) * sizeof(int), st_));
+        fac_ = zeros64(B_);
+        part_ = mem_.alloc<double>(size_t(std::max(1, nrseg_ + ncseg_)) * 5);
+        out_ = zeros64(B_ * NKKT);
+        dflags_ = mem_.alloc<uint8_t>(size_t(B_)), dact_ = mem_.alloc<uint8_t>(size_t(B_));
+        HIP_OK(hipMemsetAsync(dflags_, 0, size_t(B_), st_));
+        HIP_OK(hipMemsetAsync(dact_, 0, size_t(B_), st_));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hflags_), size_t(B_), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hpar_), size_t(2 * B_) * sizeof(T), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hout_), size_t(B_) * NKKT * sizeof(double), hipHostMallocDefault));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+
+    // Block tables of both sides for stage size `cap` (device copies; the previous ones stay allocated).
+    void set_cap(int cap) {
+        auto table = [&](const Pool& pool) {
+            std::vector<std::vector<int32_t>> per(pool.rows.size());
+            for (size_t u = 0; u < pool.rows.size(); ++u) {
+                // local row pointers of distinct matrix u
+                std::vector<int32_t> lp(static_cast<size_t>(pool.rows[u]) + 1);
+                const int32_t b0 = pool.ptr[size_t(pool.pbase[u])];
+                for (int r = 0; r <= pool.rows[u]; ++r) lp[size_t(r)] = pool.ptr[size_t(pool.pbase[u]) + size_t(r)] - b0;
+                per[u] = row_blocks_cap(pool.rows[u], lp.data(), cap);
+            }
+            std::vector<int32_t> t;
+            for (int i = 0; i < B_; ++i) {
+                const std::vector<int32_t>& b = per[size_t(mat_of_[size_t(i)])];
+                for (size_t k = 0; k + 1 < b.size(); ++k) t.push_back(i), t.push_back(b[k]), t.push_back(b[k + 1]);
+            }
+            return t;
+        };
+        const std::vector<int32_t> ta = table(PA_), tt = table(PT_);
+        auto it = tables_.find(cap);
+        if (it == tables_.end()) {
+            it = tables_.emplace(cap, std::make_pair(up_int(ta), up_int(tt))).first;
+        }
+        A_.blk = A64_.blk = it->second.first, AT_.blk = AT64_.blk = it->second.second;
+        A_.nblk = A64_.nblk = int(ta.size() / 3), AT_.nblk = AT64_.nblk = int(tt.size() / 3);
+        cap_ = cap;
+        if (graph_) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+        }
+    }
+
+    // Times a few iteration blocks for each stage size and keeps the fastest (all instances active).
+    void autotune() {
+        std::vector<uint8_t> all(static_cast<size_t>(B_), 1);
+        const bool saved_graph = o_.use_graph;
+        o_.use_graph = false;
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        float best = 0;
+        int best_cap = kCap;
+        for (int cap : {1024, 2048, 4096}) {
+            set_cap(cap);
+            upload_act(all);
+            launch_block(4);  // warm-up
+            HIP_OK(hipEventRecord(e0, st_));
+            for (int r = 0; r < 3; ++r) launch_block(8);
+            HIP_OK(hipEventRecord(e1, st_));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            tune_ms_.push_back(ms);
+            if (cap == 1024 || ms < best) best = ms, best_cap = cap;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        o_.use_graph = saved_graph;
+        set_cap(best_cap);
+        HIP_OK(hipMemsetAsync(kc_, 0, size_t(B_) * sizeof(int), st_));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+
+    Seg rseg() const { return Seg{d_rtab_, d_roff_, d_m_}; }
+    Seg cseg() const { return Seg{d_ctab_, d_coff_, d_n_}; }
+
+    void upload_flags(const std::vector<uint8_t>& on) {
+        std::copy(on.begin(), on.end(), hflags_);
+        HIP_OK(hipMemcpyAsync(dflags_, hflags_, size_t(B_), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+    void upload_act(const std::vector<uint8_t>& on) {
+        std::copy(on.begin(), on.end(), hflags_);
+        HIP_OK(hipMemcpyAsync(dact_, hflags_, size_t(B_), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+
+    template <class S, class U, class Epi>
+    void spmv_rows(const S& side, const U* v, const Epi& epi, const uint8_t* on) {
+        if (side.nblk == 0) return;
+        switch (cap_) {
+            case 1024:
+                k_bblocks<U, 1024, Epi><<<side.nblk, kWG, 0, st_>>>(side.nblk, side.blk, side.pbase, side.ptr, side.idx,
+                                                                   side.val, side.voff, v, on, epi);
+                break;
+            case 4096:
+                k_bblocks<U, 4096, Epi><<<side.nblk, kWG, 0, st_>>>(side.nblk, side.blk, side.pbase, side.ptr, side.idx,
+                                                                   side.val, side.voff, v, on, epi);
+                break;
+            default:
+                k_bblocks<U, 2048, Epi><<<side.nblk, kWG, 0, st_>>>(side.nblk, side.blk, side.pbase, side.ptr, side.idx,
+                                                                   side.val, side.voff, v, on, epi);
+        }
+    }
+    BEpiHCol<T> col_epi(int s, int mode, bool store) {
+        return BEpiHCol<T>{x_, xp_, w_, store ? ATz_ : nullptr, x0_, c_in_, l_in_, u_in_, par_, kc_, d_coff_, s, mode,
+                           T(o_.rho), 0, T(0), T(0)};
+    }
+    BEpiHRow<T> row_epi(int s) {
+        return BEpiHRow<T>{z_, z0_, zh_, rl_in_, ru_in_, par_, kc_, d_roff_, s, T(o_.rho), 0, T(0), T(0)};
+    }
+    hipEvent_t event(size_t i) {
+        while (events_.size() <= i) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            events_.push_back(e);
+        }
+        return events_[i];
+    }
+    void launch_block(int steps) {
+        for (int s = 0; s < steps; ++s) {
+            const BEpiHRow<T> er = row_epi(s);
+            const BEpiHCol<T> ec = col_epi(s, 0, s + 1 == steps);
+            if (profile_) {
+                const size_t e = pending_;
+                HIP_OK(hipEventRecord(event(e), st_));
+                spmv_rows(A_, w_, er, dact_);
+                HIP_OK(hipEventRecord(event(e + 1), st_));
+                spmv_rows(AT_, z_, ec, dact_);
+                HIP_OK(hipEventRecord(event(e + 2), st_));
+                pending_ += 3;
+            } else {
+                spmv_rows(A_, w_, er, dact_);
+                spmv_rows(AT_, z_, ec, dact_);
+            }
+        }
+        k_bset_counters<<<(B_ + kWG - 1) / kWG, kWG, 0, st_>>>(B_, dact_, kc_, steps, 0);
+    }
+    void collect_events() {
+        if (!pending_) return;
+        HIP_OK(hipStreamSynchronize(st_));
+        for (size_t i = 0; i + 2 < pending_; i += 3) {
+            float a = 0, b = 0;
+            HIP_OK(hipEventElapsedTime(&a, events_[i], events_[i + 1]));
+            HIP_OK(hipEventElapsedTime(&b, events_[i + 1], events_[i + 2]));
+            times_.row_ms += a, times_.col_ms += b;
+            times_.row_launches++, times_.col_launches++;
+        }
+        pending_ = 0;
+    }
+
+    int B_;
+    BatchOptions o_;
+    bool profile_ = false;
+    Mem mem_;
+    hipStream_t st_ = nullptr;
+    hipGraphExec_t graph_ = nullptr;
+    std::vector<int32_t> m_, n_, roff_, coff_, mat_of_;
+    int mtot_ = 0, ntot_ = 0;
+    int64_t nnz_total_ = 0;
+    Pool PA_, PT_;
+    std::vector<int32_t> rtab_, ctab_, rfirst_, rcount_, cfirst_, ccount_;
+    int nrseg_ = 0, ncseg_ = 0, cap_ = kCap;
+    std::map<int, std::pair<int*, int*>> tables_;
+    std::vector<float> tune_ms_;
+    Side A_, AT_;
+    Side64 A64_, AT64_;
+    int *d_roff_ = nullptr, *d_coff_ = nullptr, *d_m_ = nullptr, *d_n_ = nullptr;
+    int *d_rtab_ = nullptr, *d_ctab_ = nullptr, *d_rfirst_ = nullptr, *d_rcount_ = nullptr;
+    int *d_cfirst_ = nullptr, *d_cfirst5_ = nullptr, *d_ccount_ = nullptr;
+    double *c64_ = nullptr, *l64_ = nullptr, *u64_ = nullptr, *Dc64_ = nullptr, *rl64_ = nullptr, *ru64_ = nullptr,
+           *Dr64_ = nullptr;
+    double *xb_ = nullptr, *zb_ = nullptr, *Axb_ = nullptr, *ATzb_ = nullptr, *buf64_ = nullptr, *fac_ = nullptr;
+    T *c_in_ = nullptr, *l_in_ = nullptr, *u_in_ = nullptr, *rl_in_ = nullptr, *ru_in_ = nullptr, *zh_ = nullptr;
+    T *x_ = nullptr, *xp_ = nullptr, *w_ = nullptr, *x0_ = nullptr, *ATz_ = nullptr, *z_ = nullptr, *z0_ = nullptr,
+      *Ax_ = nullptr;
+    T *pv_ = nullptr, *pw_ = nullptr, *pu_ = nullptr, *par_ = nullptr, *hpar_ = nullptr;
+    int* kc_ = nullptr;
+    double *part_ = nullptr, *out_ = nullptr, *hout_ = nullptr;
+    uint8_t *dflags_ = nullptr, *dact_ = nullptr, *hflags_ = nullptr;
+    size_t mem_used_ = 0;
+    std::string name_;
+    std::vector<hipEvent_t> events_;
+    size_t pending_ = 0;
+    KernelTimes times_;
+};
+
+}  // namespace
+
+std::unique_ptr<BatchDevice> make_batch_device(const std::vector<const Scaled*>& probs, const BatchOptions& opt) {
+    for (const Scaled* p : probs)
+        if (!p) throw std::invalid_argument("null batch instance");
+    if (opt.precision_bits == 32) return std::unique_ptr<BatchDevice>(new BatchDev<float>(probs, opt));
+    if (opt.precision_bits == 64) return std::unique_ptr<BatchDevice>(new BatchDev<double>(probs, opt));
+    throw std::runtime_error("precision must be 32 or 64 bits");
+}
+
+}  // namespace tpdhg

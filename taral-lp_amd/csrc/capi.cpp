@@ -23,6 +23,10 @@ struct tpdhg_dev {
 struct tpdhg_fast {
     std::unique_ptr<tpdhg::FastDevice> d;
 };
+struct tpdhg_batch {
+    std::unique_ptr<tpdhg::BatchSession> s;
+    tpdhg::Options o;
+};
 struct tpdhg_mlp {
     std::unique_ptr<tpdhg::Mlp> net;
 };
@@ -426,6 +430,74 @@ int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const 
         return TPDHG_OK;
     });
 }
+
+int tpdhg_batch_create(const tpdhg_model* const* models, int32_t count, const tpdhg_options* opt, int32_t cap,
+                       tpdhg_batch** out) {
+    if (!models || !out || count <= 0) return fail(TPDHG_ERR_ARG, "null argument or empty batch");
+    tpdhg::Options o = to_options(opt);
+    if (o.eval_freq < 1) o.eval_freq = 1;
+    if (int rc = check_options(o)) return rc;
+    if (!o.halpern) return fail(TPDHG_ERR_ARG, "batches run the fast path: set halpern = 1");
+    if (cap != 0 && cap != 1024 && cap != 2048 && cap != 4096) return fail(TPDHG_ERR_ARG, "cap must be 0, 1024, 2048 or 4096");
+    std::vector<const tpdhg::LP*> lps;
+    for (int32_t i = 0; i < count; ++i) {
+        if (!models[i]) return fail(TPDHG_ERR_ARG, "null model in batch");
+        if (const char* why = tpdhg::input_error(models[i]->lp)) return fail(TPDHG_ERR_UNSUPPORTED, why);
+        lps.push_back(&models[i]->lp);
+    }
+    return guarded([&]() -> int {
+        auto b = std::make_unique<tpdhg_batch>();
+        b->o = o;
+        b->s = std::make_unique<tpdhg::BatchSession>(lps, o, cap);
+        *out = b.release();
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_batch_get_info(const tpdhg_batch* b, tpdhg_batch_info* info) {
+    if (!b || !info) return fail(TPDHG_ERR_ARG, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    tpdhg::BatchDevice& d = b->s->device();
+    info->count = d.count(), info->unique_matrices = d.unique_matrices(), info->cap = d.cap();
+    info->nnz_total = d.nnz_total(), info->nnz_stored = d.nnz_stored();
+    info->device_mem_bytes = int64_t(d.mem_used()), info->setup_s = b->s->setup_s();
+    const std::vector<float> t = d.tune_ms();
+    for (size_t k = 0; k < t.size() && k < 3; ++k) info->tune_ms[k] = t[k];
+    return TPDHG_OK;
+}
+
+int tpdhg_batch_set_vectors(tpdhg_batch* b, int32_t i, const double* cost, const double* col_lo, const double* col_up,
+                            const double* row_lo, const double* row_up) {
+    if (!b || !cost || !col_lo || !col_up || !row_lo || !row_up) return fail(TPDHG_ERR_ARG, "null argument");
+    if (i < 0 || i >= b->s->count()) return fail(TPDHG_ERR_ARG, "instance index out of range");
+    return guarded([&]() -> int {
+        b->s->set_vectors(i, cost, col_lo, col_up, row_lo, row_up);
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_batch_solve(tpdhg_batch* b, const double* const* x0, const double* const* y0, const double* primal_weight,
+                      tpdhg_result* results, double* const* x, double* const* y) {
+    if (!b || !results) return fail(TPDHG_ERR_ARG, "null argument");
+    const int B = b->s->count();
+    if (primal_weight)
+        for (int i = 0; i < B; ++i)
+            if (!(primal_weight[i] >= 0) || !std::isfinite(primal_weight[i]))
+                return fail(TPDHG_ERR_ARG, "primal weights must be finite and >= 0");
+    return guarded([&]() -> int {
+        std::vector<const double*> xs, ys;
+        std::vector<double> om;
+        if (x0) xs.assign(x0, x0 + B);
+        if (y0) ys.assign(y0, y0 + B);
+        if (primal_weight) om.assign(primal_weight, primal_weight + B);
+        std::vector<tpdhg::Report> reps = b->s->solve(xs, ys, om);
+        for (int i = 0; i < B; ++i)
+            fill_result(reps[size_t(i)], b->s->lp(i), &results[i], x ? x[i] : nullptr, y ? y[i] : nullptr);
+        return TPDHG_OK;
+    });
+}
+
+void tpdhg_batch_free(tpdhg_batch* b) { delete b; }
 
 int tpdhg_mlp_create(int32_t n_in, int32_t h1, int32_t h2, int32_t n_out, uint64_t seed, tpdhg_mlp** out) {
     if (!out) return fail(TPDHG_ERR_ARG, "null argument");

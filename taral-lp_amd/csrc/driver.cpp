@@ -272,6 +272,26 @@ Session::Session(const LP& lp, const Options& o, Timings tm) : lp_(lp), o_(o), t
     setup_s_ = tm_.parse + tm_.scale + tm_.init + tm_.transfer;
 }
 
+// Unscale and check the point on the ORIGINAL model in FP64 (gpu/pdhg.cu:866-876).
+void finish_report(const LP& lp, const Scaled& P, Report& rep) {
+    rep.x.resize(size_t(lp.n)), rep.y.resize(size_t(lp.m));
+    for (int j = 0; j < lp.n; ++j) rep.x[size_t(j)] = rep.out.x[size_t(j)] * P.Dc[size_t(j)];
+    for (int i = 0; i < lp.m; ++i) rep.y[size_t(i)] = P.Dr[size_t(i)] * rep.out.z[size_t(i)];
+    long double obj = lp.obj_const;
+    for (int j = 0; j < lp.n; ++j) obj += (long double)lp.cost[size_t(j)] * rep.x[size_t(j)];
+    std::vector<double> ax(static_cast<size_t>(lp.m), 0.0);
+    double bviol = 0, rviol = 0;
+    for (int j = 0; j < lp.n; ++j) {
+        const double xj = rep.x[size_t(j)];
+        for (int64_t k = lp.colptr[size_t(j)]; k < lp.colptr[size_t(j) + 1]; ++k)
+            ax[size_t(lp.rowidx[size_t(k)])] += lp.val[size_t(k)] * xj;
+        bviol = std::max(bviol, std::max(lp.col_lo[size_t(j)] - xj, xj - lp.col_up[size_t(j)]));
+    }
+    for (int i = 0; i < lp.m; ++i)
+        rviol = std::max(rviol, std::max(lp.row_lo[size_t(i)] - ax[size_t(i)], ax[size_t(i)] - lp.row_up[size_t(i)]));
+    rep.objective = double(obj), rep.max_row_violation = rviol, rep.max_bound_violation = bviol;
+}
+
 Report Session::solve(const double* x0_orig, const double* y0_orig) {
     const LP& lp = lp_;
     const Scaled& P = P_;
@@ -307,23 +327,7 @@ Report Session::solve(const double* x0_orig, const double* y0_orig) {
     norm_a_ = rep.out.norm_a;  // A_s is fixed for the session's lifetime: later solves skip the power iteration
     rep.out.warm_used = warm, rep.out.warm_kkt = warm_kkt, rep.out.cold_kkt = cold_kkt;
     rep.solve_s = since(t_call);
-    // Unscale and check the point on the ORIGINAL model in FP64 (gpu/pdhg.cu:866-876).
-    rep.x.resize(size_t(lp.n)), rep.y.resize(size_t(lp.m));
-    for (int j = 0; j < lp.n; ++j) rep.x[size_t(j)] = rep.out.x[size_t(j)] * P.Dc[size_t(j)];
-    for (int i = 0; i < lp.m; ++i) rep.y[size_t(i)] = P.Dr[size_t(i)] * rep.out.z[size_t(i)];
-    long double obj = lp.obj_const;
-    for (int j = 0; j < lp.n; ++j) obj += (long double)lp.cost[size_t(j)] * rep.x[size_t(j)];
-    std::vector<double> ax(static_cast<size_t>(lp.m), 0.0);
-    double bviol = 0, rviol = 0;
-    for (int j = 0; j < lp.n; ++j) {
-        const double xj = rep.x[size_t(j)];
-        for (int64_t k = lp.colptr[size_t(j)]; k < lp.colptr[size_t(j) + 1]; ++k)
-            ax[size_t(lp.rowidx[size_t(k)])] += lp.val[size_t(k)] * xj;
-        bviol = std::max(bviol, std::max(lp.col_lo[size_t(j)] - xj, xj - lp.col_up[size_t(j)]));
-    }
-    for (int i = 0; i < lp.m; ++i)
-        rviol = std::max(rviol, std::max(lp.row_lo[size_t(i)] - ax[size_t(i)], ax[size_t(i)] - lp.row_up[size_t(i)]));
-    rep.objective = double(obj), rep.max_row_violation = rviol, rep.max_bound_violation = bviol;
+    finish_report(lp, P, rep);
     rep.total_s = setup_s_ + since(t_call);
     return rep;
 }
@@ -350,6 +354,248 @@ void Session::set_vectors(LP& lp, const double* cost, const double* col_lo, cons
                   lp.maximize ? -1.0 : 1.0);
     if (fast_) fast_->set_vectors(P_.c.data(), P_.l.data(), P_.u.data(), P_.rl.data(), P_.ru.data());
     else dev_->set_vectors(P_.c.data(), P_.l.data(), P_.u.data(), P_.rl.data(), P_.ru.data());
+}
+
+std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>& P, const Options& o,
+                               double time_budget, const std::vector<const double*>& x0,
+                               const std::vector<const double*>& z0, const std::vector<double>& omega0,
+                               std::vector<double>* norms) {
+    const int B = b.count();
+    std::vector<Outcome> r(static_cast<size_t>(B));
+    auto t0 = Clock::now(), t_progress = t0;
+    b.set_profile(o.profile_kernels);
+    std::vector<double> na = norms && int(norms->size()) == B ? *norms : b.power_norm(200);
+    if (norms) *norms = na;
+    const double power_s = since(t0);
+    std::vector<double> eta(static_cast<size_t>(B)), w(static_cast<size_t>(B)), tau(static_cast<size_t>(B)),
+        sig(static_cast<size_t>(B));
+    for (int i = 0; i < B; ++i) {
+        const Scaled& Q = *P[size_t(i)];
+        r[size_t(i)].norm_a = na[size_t(i)], r[size_t(i)].power_s = power_s;
+        eta[size_t(i)] = na[size_t(i)] > 0 ? 0.95 / na[size_t(i)] : 1.0;
+        double cs = 0, bs = 0;  // as run_fast
+        for (double v : Q.c) cs += v * v;
+        for (int k = 0; k < Q.m; ++k) {
+            const double v = row_bound_measure(Q.rl[size_t(k)], Q.ru[size_t(k)]);
+            bs += v * v;
+        }
+        cs = std::sqrt(cs), bs = std::sqrt(bs);
+        w[size_t(i)] = cs > 1e-10 && bs > 1e-10 ? cs / bs : 1.0;
+        if (size_t(i) < omega0.size() && omega0[size_t(i)] > 0) w[size_t(i)] = omega0[size_t(i)];
+    }
+    auto push_steps = [&]() {
+        for (int i = 0; i < B; ++i) tau[size_t(i)] = eta[size_t(i)] / w[size_t(i)], sig[size_t(i)] = eta[size_t(i)] * w[size_t(i)];
+        b.set_steps(tau, sig);
+    };
+    auto kslot = [](std::vector<double>& kk, int i) { return kk.data() + size_t(i) * NKKT; };
+    auto wkkt = [&](const double* k, double wi) {
+        const double g = k[POBJ] - (k[DROW] + k[DCOL]);
+        return std::sqrt(wi * k[PR2S] + k[DR2S] / wi + g * g);
+    };
+    b.set_base(x0, z0);
+    std::vector<uint8_t> all(static_cast<size_t>(B), 1), active(static_cast<size_t>(B), 0);
+    std::vector<double> kk(size_t(B) * NKKT, 0.0), k_last(static_cast<size_t>(B)), k_prev(static_cast<size_t>(B), kInf);
+    std::vector<int64_t> n_since(static_cast<size_t>(B), 0), it_of(static_cast<size_t>(B), 0);
+    std::vector<Rel> best(static_cast<size_t>(B), Rel{kInf, kInf, kInf});
+    b.begin_round(all);
+    push_steps();
+    b.restart(all);
+    b.eval(all, kk);
+    int live = 0;
+    for (int i = 0; i < B; ++i) {
+        best[size_t(i)] = rel_of(kslot(kk, i), *P[size_t(i)], o.row_rel);
+        k_last[size_t(i)] = wkkt(kslot(kk, i), w[size_t(i)]);
+        if (best[size_t(i)].worst() <= o.tol) r[size_t(i)].status = "near_optimal";
+        else active[size_t(i)] = 1, ++live;
+    }
+    int64_t it = 0;
+    std::vector<uint8_t> confirm(static_cast<size_t>(B)), rs_move(static_cast<size_t>(B)), rs_any(static_cast<size_t>(B));
+    while (live > 0) {
+        const int steps = int(std::min<int64_t>(o.eval_freq, o.max_iter - it));
+        if (steps <= 0) break;
+        b.iterate(steps, active);
+        it += steps;
+        b.eval(active, kk);
+        std::fill(confirm.begin(), confirm.end(), 0), std::fill(rs_move.begin(), rs_move.end(), 0);
+        std::fill(rs_any.begin(), rs_any.end(), 0);
+        bool any_confirm = false, any_move = false, any_rs = false;
+        const bool out_of_time = since(t0) > time_budget;
+        for (int i = 0; i < B; ++i) {
+            if (!active[size_t(i)]) continue;
+            Outcome& q = r[size_t(i)];
+            n_since[size_t(i)] += steps, it_of[size_t(i)] = it;
+            const double* k = kslot(kk, i);
+            const Rel rc = rel_of(k, *P[size_t(i)], o.row_rel);
+            best[size_t(i)] = rc;
+            if (!std::isfinite(rc.worst())) {
+                q.status = "numerical_failure", active[size_t(i)] = 0, --live;
+                continue;
+            }
+            if (rc.worst() <= o.tol) {
+                confirm[size_t(i)] = 1, any_confirm = true;
+                continue;
+            }
+            if (out_of_time) {
+                q.status = "time_limit", active[size_t(i)] = 0, --live;
+                continue;
+            }
+            if (it >= o.max_iter) {
+                active[size_t(i)] = 0, --live;
+                continue;
+            }
+            const double kc = wkkt(k, w[size_t(i)]);
+            const bool restart = double(n_since[size_t(i)]) >= 0.36 * double(it) || kc <= 0.2 * k_last[size_t(i)] ||
+                                 (kc <= 0.8 * k_last[size_t(i)] && kc > k_prev[size_t(i)]);
+            k_prev[size_t(i)] = kc;
+            if (restart) {
+                const double dx = std::sqrt(k[DX2]), dz = std::sqrt(k[DZ2]);
+                if (dx > 1e-10 && dz > 1e-10)
+                    w[size_t(i)] = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w[size_t(i)]));
+                if (o.refine && q.refinements < o.max_refinements) rs_move[size_t(i)] = 1, any_move = true, q.refinements++;
+                rs_any[size_t(i)] = 1, any_rs = true, q.restarts++;
+            }
+        }
+        if (any_confirm) {  // FP64 confirmation: recentre at the point and evaluate there
+            b.end_round(confirm);
+            b.begin_round(confirm);
+            push_steps();
+            b.restart(confirm);
+            b.eval(confirm, kk);
+            for (int i = 0; i < B; ++i) {
+                if (!confirm[size_t(i)]) continue;
+                best[size_t(i)] = rel_of(kslot(kk, i), *P[size_t(i)], o.row_rel);
+                if (best[size_t(i)].worst() <= o.tol) {
+                    r[size_t(i)].status = "near_optimal", active[size_t(i)] = 0, --live;
+                    continue;
+                }
+                r[size_t(i)].refinements++;
+                k_last[size_t(i)] = wkkt(kslot(kk, i), w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
+            }
+        }
+        if (any_rs) {
+            if (any_move) {
+                b.end_round(rs_move);
+                b.begin_round(rs_move);
+            }
+            push_steps();
+            b.restart(rs_any);
+            for (int i = 0; i < B; ++i)
+                if (rs_any[size_t(i)])
+                    k_last[size_t(i)] = wkkt(kslot(kk, i), w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
+        }
+        if ((o.verbose && (it / o.eval_freq) % 50 == 0) || progress_due(o, t_progress)) {
+            double worst = 0;
+            for (int i = 0; i < B; ++i)
+                if (active[size_t(i)]) worst = std::max(worst, best[size_t(i)].worst());
+            std::fprintf(stderr, "batch it %ld  live %d of %d  worst live rel KKT %.2e\n", long(it), live, B, worst);
+        }
+    }
+    const double iterate_s = since(t0) - power_s;
+    for (int i = 0; i < B; ++i) {
+        Outcome& q = r[size_t(i)];
+        q.iterations = it_of[size_t(i)];
+        q.rel_primal = best[size_t(i)].p, q.rel_dual = best[size_t(i)].d, q.rel_gap = best[size_t(i)].g;
+        q.omega = w[size_t(i)], q.from_average = false, q.iterate_s = iterate_s;
+        q.x.resize(size_t(P[size_t(i)]->n)), q.z.resize(size_t(P[size_t(i)]->m));
+        b.download(i, q.x.data(), q.z.data());
+    }
+    const KernelTimes kt = b.kernel_times();
+    for (Outcome& q : r) q.kernels = kt;
+    return r;
+}
+
+BatchSession::BatchSession(const std::vector<const LP*>& lps, const Options& o, int cap) : o_(o) {
+    auto t = Clock::now();
+    if (lps.empty()) throw std::invalid_argument("empty batch");
+    lps_.reserve(lps.size());
+    for (const LP* lp : lps) {
+        if (!lp) throw std::invalid_argument("null batch instance");
+        lps_.push_back(*lp);
+    }
+    P_.reserve(lps_.size());
+    for (const LP& lp : lps_)
+        P_.push_back(build_scaled_csc(lp.m, lp.n, lp.colptr.data(), lp.rowidx.data(), lp.val.data(), lp.cost.data(),
+                                      lp.col_lo.data(), lp.col_up.data(), lp.row_lo.data(), lp.row_up.data(),
+                                      lp.maximize ? -1.0 : 1.0));
+    if (device_count() == 0)
+        throw std::runtime_error("no HIP device visible: the PDHG solver runs on an AMD GPU only (there is no CPU path)");
+    BatchOptions bo;
+    bo.precision_bits = o.precision_bits, bo.eval_freq = o.eval_freq, bo.use_graph = o.graph, bo.rho = o.rho;
+    bo.cap = cap;
+    std::vector<const Scaled*> ps;
+    for (const Scaled& P : P_) ps.push_back(&P);
+    dev_ = make_batch_device(ps, bo);
+    setup_s_ = since(t);
+}
+
+std::vector<Report> BatchSession::solve(const std::vector<const double*>& x0, const std::vector<const double*>& y0,
+                                        const std::vector<double>& omega0) {
+    auto t_call = Clock::now();
+    const int B = count();
+    std::vector<std::vector<double>> xs(static_cast<size_t>(B)), zs(static_cast<size_t>(B));
+    std::vector<const double*> xw(static_cast<size_t>(B), nullptr), zw(static_cast<size_t>(B), nullptr);
+    for (int i = 0; i < B; ++i) {
+        const LP& lp = lps_[size_t(i)];
+        const Scaled& P = P_[size_t(i)];
+        if (size_t(i) < x0.size() && x0[size_t(i)]) {
+            xs[size_t(i)].resize(size_t(lp.n));
+            for (int j = 0; j < lp.n; ++j) xs[size_t(i)][size_t(j)] = x0[size_t(i)][j] / P.Dc[size_t(j)];
+            xw[size_t(i)] = xs[size_t(i)].data();
+        }
+        if (size_t(i) < y0.size() && y0[size_t(i)]) {
+            zs[size_t(i)].resize(size_t(lp.m));
+            for (int k = 0; k < lp.m; ++k) zs[size_t(i)][size_t(k)] = y0[size_t(i)][k] / P.Dr[size_t(k)];
+            zw[size_t(i)] = zs[size_t(i)].data();
+        }
+    }
+    std::vector<uint8_t> warm(static_cast<size_t>(B), 0);
+    std::vector<double> wk(static_cast<size_t>(B), 0.0), ck(static_cast<size_t>(B), 0.0);
+    bool any_warm = false;
+    for (int i = 0; i < B; ++i) warm[size_t(i)] = xw[size_t(i)] || zw[size_t(i)], any_warm = any_warm || warm[size_t(i)];
+    if (any_warm && o_.warm_fallback) {  // relative KKT of the warm and of the cold start, all instances at once
+        std::vector<uint8_t> all(static_cast<size_t>(B), 1);
+        std::vector<double> kk;
+        auto probe = [&](const std::vector<const double*>& bx, const std::vector<const double*>& bz,
+                         std::vector<double>& dst) {
+            dev_->set_base(bx, bz);
+            dev_->begin_round(all);
+            dev_->eval(all, kk);
+            for (int i = 0; i < B; ++i) dst[size_t(i)] = rel_of(kk.data() + size_t(i) * NKKT, P_[size_t(i)], o_.row_rel).worst();
+        };
+        probe(xw, zw, wk);
+        probe({}, {}, ck);
+        for (int i = 0; i < B; ++i)
+            if (warm[size_t(i)] && !(wk[size_t(i)] < ck[size_t(i)])) warm[size_t(i)] = 0, xw[size_t(i)] = zw[size_t(i)] = nullptr;
+    }
+    std::vector<double> om(static_cast<size_t>(B), 0.0);
+    for (int i = 0; i < B; ++i)
+        if (warm[size_t(i)] && size_t(i) < omega0.size()) om[size_t(i)] = omega0[size_t(i)];
+    std::vector<const Scaled*> ps;
+    for (const Scaled& P : P_) ps.push_back(&P);
+    std::vector<Outcome> outs = run_batch(*dev_, ps, o_, o_.time_limit, xw, zw, om, &norms_);
+    const double solve_s = since(t_call);
+    std::vector<Report> reps(static_cast<size_t>(B));
+    for (int i = 0; i < B; ++i) {
+        Report& rep = reps[size_t(i)];
+        rep.out = std::move(outs[size_t(i)]);
+        rep.out.warm_used = warm[size_t(i)] != 0, rep.out.warm_kkt = wk[size_t(i)], rep.out.cold_kkt = ck[size_t(i)];
+        rep.setup_s = setup_s_, rep.solve_s = solve_s, rep.total_s = setup_s_ + solve_s;
+        rep.device_name = dev_->name(), rep.device_mem_bytes = dev_->mem_used(), rep.nnz = P_[size_t(i)].A.nnz();
+        finish_report(lps_[size_t(i)], P_[size_t(i)], rep);
+    }
+    return reps;
+}
+
+void BatchSession::set_vectors(int i, const double* cost, const double* col_lo, const double* col_up,
+                               const double* row_lo, const double* row_up) {
+    if (i < 0 || i >= count()) throw std::invalid_argument("instance index out of range");
+    LP& lp = lps_[size_t(i)];
+    lp.cost.assign(cost, cost + lp.n), lp.col_lo.assign(col_lo, col_lo + lp.n), lp.col_up.assign(col_up, col_up + lp.n);
+    lp.row_lo.assign(row_lo, row_lo + lp.m), lp.row_up.assign(row_up, row_up + lp.m);
+    Scaled& P = P_[size_t(i)];
+    scale_vectors(P, lp.cost.data(), lp.col_lo.data(), lp.col_up.data(), lp.row_lo.data(), lp.row_up.data(),
+                  lp.maximize ? -1.0 : 1.0);
+    dev_->set_vectors(i, P.c.data(), P.l.data(), P.u.data(), P.rl.data(), P.ru.data());
 }
 
 Report solve_lp(const LP& lp, const Options& o, Timings tm, Clock::time_point t_start, const double* x0_orig,

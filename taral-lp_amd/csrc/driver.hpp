@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "device.hpp"
 #include "model.hpp"
 #include "prep.hpp"
@@ -118,6 +119,42 @@ private:
     double probe_kkt(const double* xs, const double* zs);  // relative KKT at a scaled-space start point
     double setup_s_ = 0, norm_a_ = 0;
     std::chrono::steady_clock::time_point created_;
+};
+
+// Original-space point, objective and violations of rep.out (gpu/pdhg.cu:866-876).
+void finish_report(const LP& lp, const Scaled& P, Report& rep);
+
+// The fast path over a batch: every instance follows run_fast's decisions (restart tests, primal weight,
+// recentring, FP64 confirmation) on its own scalars while all live instances iterate together; an instance
+// leaves the lockstep when it stops. x0/z0/omega0 per instance (empty or null entries: cold start). norms, when
+// it holds one value per instance, skips the power iteration; it receives the estimates otherwise.
+std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>& P, const Options& o,
+                               double time_budget, const std::vector<const double*>& x0,
+                               const std::vector<const double*>& z0, const std::vector<double>& omega0,
+                               std::vector<double>* norms);
+
+// Many LPs resident on the device as one batch (the batch keeps its own copies of the models).
+class BatchSession {
+public:
+    BatchSession(const std::vector<const LP*>& lps, const Options& o, int cap = 0);
+    // x0/y0 per instance in original space (null entries: cold start), primal weight per instance (0: default);
+    // with Options::warm_fallback each instance keeps its warm start only when its KKT beats its cold start's.
+    std::vector<Report> solve(const std::vector<const double*>& x0 = {}, const std::vector<const double*>& y0 = {},
+                              const std::vector<double>& omega0 = {});
+    void set_vectors(int i, const double* cost, const double* col_lo, const double* col_up, const double* row_lo,
+                     const double* row_up);
+    int count() const { return int(lps_.size()); }
+    const LP& lp(int i) const { return lps_[size_t(i)]; }
+    BatchDevice& device() { return *dev_; }
+    double setup_s() const { return setup_s_; }
+
+private:
+    std::vector<LP> lps_;
+    std::vector<Scaled> P_;
+    Options o_;
+    std::unique_ptr<BatchDevice> dev_;
+    std::vector<double> norms_;
+    double setup_s_ = 0;
 };
 
 // Scales, uploads and solves on the current HIP device; throws std::runtime_error on device failure.

@@ -35,6 +35,7 @@ EXPORTS = [
     "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
     "tpdhg_session_set_vectors", "tpdhg_session_set_primal_weight",
     "tpdhg_result_json", "tpdhg_spmv_bench",
+    "tpdhg_batch_create", "tpdhg_batch_get_info", "tpdhg_batch_set_vectors", "tpdhg_batch_solve", "tpdhg_batch_free",
     "tpdhg_mlp_create", "tpdhg_mlp_param_count", "tpdhg_mlp_set_params", "tpdhg_mlp_get_params",
     "tpdhg_mlp_forward", "tpdhg_mlp_loss_grad", "tpdhg_mlp_train", "tpdhg_mlp_destroy",
 ]
@@ -85,6 +86,12 @@ class Result(C.Structure):
                 ("nnz", C.c_int64), ("row_kernel_ms", C.c_double), ("col_kernel_ms", C.c_double),
                 ("row_launches", C.c_int64), ("col_launches", C.c_int64), ("device", C.c_char * 64),
                 ("warm_start_used", C.c_int32), ("warm_kkt", C.c_double), ("cold_kkt", C.c_double)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("count", C.c_int32), ("unique_matrices", C.c_int32), ("cap", C.c_int32),
+                ("nnz_total", C.c_int64), ("nnz_stored", C.c_int64), ("device_mem_bytes", C.c_int64),
+                ("setup_s", C.c_double), ("tune_ms", C.c_float * 3)]
 
 
 _lib = None
@@ -148,6 +155,12 @@ def lib():
         L.tpdhg_session_set_primal_weight.argtypes = [C.c_void_p, C.c_double]
         L.tpdhg_result_json.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result), C.c_char_p, C.c_int32]
         L.tpdhg_spmv_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d]
+        _pp = C.POINTER(C.c_void_p)
+        L.tpdhg_batch_create.argtypes = [_pp, C.c_int32, C.POINTER(Options), C.c_int32, C.POINTER(C.c_void_p)]
+        L.tpdhg_batch_get_info.argtypes = [C.c_void_p, C.POINTER(BatchInfo)]
+        L.tpdhg_batch_set_vectors.argtypes = [C.c_void_p, C.c_int32, _d, _d, _d, _d, _d]
+        L.tpdhg_batch_solve.argtypes = [C.c_void_p, _pp, _pp, _d, C.POINTER(Result), _pp, _pp]
+        L.tpdhg_batch_free.argtypes = [C.c_void_p]
         _f = C.POINTER(C.c_float)
         L.tpdhg_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]
         L.tpdhg_mlp_param_count.argtypes = [C.c_void_p]
@@ -468,6 +481,72 @@ def solve(model: Model, x0=None, y0=None, **opts) -> dict:
     x0, y0 = _f64(x0), _f64(y0)
     _check(lib().tpdhg_solve(model._h, C.byref(o), _p(x0), _p(y0), C.byref(r), _p(x), _p(y)))
     return _result_dict(model, o, r, x, y, i.m, i.n)
+
+
+def _ptr_array(arrays):
+    """(void* array for C, keep-alive list) from a list of float64 arrays or Nones."""
+    keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    arr = (C.c_void_p * len(keep))(*[None if a is None else a.ctypes.data for a in keep])
+    return arr, keep
+
+
+class Batch:
+    """Many LPs solved together on the GPU by the fast path (block-diagonal packing, per-instance scalars). Every
+    instance takes the decisions a single solve with the same options takes; identical matrices are stored once."""
+
+    def __init__(self, models, cap=0, **opts):
+        opts.setdefault("halpern", 1)
+        self.models = list(models)
+        self.opts = default_options(**opts)
+        self.dims = [(md.info().m, md.info().n) for md in self.models]
+        arr = (C.c_void_p * len(self.models))(*[md._h.value for md in self.models])
+        h = C.c_void_p()
+        _check(lib().tpdhg_batch_create(arr, len(self.models), C.byref(self.opts), int(cap), C.byref(h)))
+        self._h = h
+
+    def __len__(self):
+        return len(self.models)
+
+    def info(self) -> dict:
+        i = BatchInfo()
+        _check(lib().tpdhg_batch_get_info(self._h, C.byref(i)))
+        out = {k: getattr(i, k) for k, _ in BatchInfo._fields_ if k != "tune_ms"}
+        out["tune_ms"] = list(i.tune_ms)
+        return out
+
+    def set_vectors(self, i, cost, col_lo, col_up, row_lo, row_up):
+        keep = [_f64(v) for v in (cost, col_lo, col_up, row_lo, row_up)]
+        _check(lib().tpdhg_batch_set_vectors(self._h, int(i), *[_p(v) for v in keep]))
+
+    def solve(self, x0=None, y0=None, primal_weight=None, points=True) -> list:
+        B = len(self.models)
+        res = (Result * B)()
+        ax, kx = _ptr_array(x0 if x0 is not None else [None] * B)
+        ay, ky = _ptr_array(y0 if y0 is not None else [None] * B)
+        pw = None if primal_weight is None else np.ascontiguousarray(primal_weight, dtype=np.float64)
+        xs = [np.zeros(max(n, 1)) for (m, n) in self.dims] if points else None
+        ys = [np.zeros(max(m, 1)) for (m, n) in self.dims] if points else None
+        px = (C.c_void_p * B)(*[a.ctypes.data for a in xs]) if points else None
+        py = (C.c_void_p * B)(*[a.ctypes.data for a in ys]) if points else None
+        _check(lib().tpdhg_batch_solve(self._h, ax if x0 is not None else None, ay if y0 is not None else None,
+                                       _p(pw), res, px, py))
+        out = []
+        for i, (m, n) in enumerate(self.dims):
+            x = xs[i] if points else np.zeros(0)
+            y = ys[i] if points else np.zeros(0)
+            out.append(_result_dict(self.models[i], self.opts, res[i], x, y, m, n))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.tpdhg_batch_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _f32p = C.POINTER(C.c_float)

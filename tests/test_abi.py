@@ -116,3 +116,22 @@ def test_options_carry_warm_start_fields():
     o = ffi.default_options()
     assert o.primal_weight == 0 and o.warm_fallback == 0
     assert ffi.Result().warm_start_used == 0
+
+
+def test_batch_create_validates_before_the_device():
+    from taral_pdhg import gen
+    md = ffi.Model.from_lp(gen.planted_lp(20, 40, 3, seed=0))
+    with pytest.raises(ffi.TpdhgError) as e:
+        ffi.Batch([md], halpern=0)
+    assert e.value.code == ffi.ERR_ARG and "halpern" in str(e.value)
+    with pytest.raises(ffi.TpdhgError) as e:
+        ffi.Batch([md], cap=3000)
+    assert e.value.code == ffi.ERR_ARG
+    mip = ffi.Model.from_mps(os.path.join(os.path.dirname(__file__), "golden", "mps", "gate_intorg.mps"))
+    with pytest.raises(ffi.TpdhgError) as e:
+        ffi.Batch([md, mip])
+    assert e.value.code == ffi.ERR_UNSUPPORTED
+    if ffi.device_count() == 0:
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.Batch([md, md])
+        assert e.value.code == ffi.ERR_NO_DEVICE
