@@ -1,24 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: LP solves/sec of the restarted PDHG solver on one MI355X, with the SpMV-kernel roofline.
+"""Benchmark: LP solves/sec of the restarted PDHG solver on one MI355X, with the iteration-kernel roofline.
 
-Workload (default): the single large planted LP — m = 1,000,000 equality rows, n = 2,000,000 columns,
-20 nonzeros per column at uniform-random rows, N(0,1) values, 0 <= x <= 10, seed 1 — solved to 1e-6
-relative KKT (original-space primal residual, dual residual and gap, the reference's stopping test).
-A "step" is one full solve from a cold start of a problem already resident in HBM (power iteration,
-restarted PDHG iterations, FP64 evaluations, download and the original-space check). The default mode is the
-fast path (FP32 reflected Halpern PDHG iterates inside FP64 iterative refinement); one config-1 solve to
-1e-6 takes a few minutes on MI355X because this LP family needs ~470k iterations, so the defaults time one
-step without warm-up.
+Workloads (--workload):
+  cfg1 (default)  the single large planted LP: m = 1,000,000 equality rows, n = 2,000,000 columns, 20 nonzeros per
+                  column at uniform-random rows, N(0,1) values, 0 <= x <= 10, seed 1.
+  cfg2            the block-angular planning LP (64 blocks of 5,000 x 8,000 plus 200 linking rows), planted.
+  cfg3            a batch of 512 independent planted LPs of uneven size, solved together by the batched fast path.
+  cfg0, mid       the 2,000 x 5,000 and 100,000 x 200,000 planted LPs.
+Every solve runs to 1e-6 relative KKT (original-space primal residual, dual residual and gap, the reference's
+stopping test). A "step" is one full solve from a cold start of problems already resident in HBM (power iteration,
+PDHG iterations, FP64 evaluations, download and the original-space check); for cfg3 one step solves the whole
+batch and the value counts every LP. The default mode is the fast path (FP32 reflected Halpern PDHG iterates inside
+FP64 iterative refinement); one cfg1 solve takes minutes, so the defaults time one step without warm-up.
 
 Printed JSON line: the contract's fields, plus
-  roofline: the dominant iteration kernel (A^T z fused with the primal step, or A x fused with the dual
-            step): algorithmic bytes per launch / mean launch time from HIP events on the solver stream
-            during the timed solves, against the 8 TB/s HBM peak;
-  cpu_baseline: the CPU oracle (one thread, FP64) timed on a bounded sample of the same workload (a few
-            PDHG iterations), scaled to solves/sec with the GPU solve's iteration count;
-  fp64_reference: the plain FP64 cold start of the reference algorithm on the same LP, capped at
-            --fp64-time-limit seconds (status, iterations and relative KKT at the cap; fp64_solves_per_s and
-            speedup_vs_fp64 only when it converges within the cap).
+  roofline: single LPs: the dominant iteration kernel (A^T z fused with the primal step, or A x fused with the dual
+            step), algorithmic bytes per launch / mean launch time from HIP events on the solver stream in a
+            separate capped solve; batches: one batched iteration with every instance live (both launches),
+            timed by the batch's own stage autotuner; against the 8 TB/s HBM peak;
+  cpu_baseline: the CPU oracle (one thread, FP64) timed on a bounded sample (a few PDHG iterations), scaled to
+            solves/sec with the GPU solve's iteration counts;
+  fp64_reference: the plain FP64 cold start of the reference algorithm (single LPs: the same LP, capped at
+            --fp64-time-limit; batches: the first --fp64-sample LPs one at a time).
 """
 import argparse
 import json
@@ -34,10 +37,19 @@ for p in (ROOT, os.path.join(ROOT, "taral-lp_amd")):
 HBM_PEAK_GBS = 8000.0
 METRIC = "LP solves/sec to 1e-6 rel KKT (learned+FP32 vs plain FP64 PDHG); SpMV HBM GB/s"
 
-WORKLOADS = {  # name: (m, n, nnz per column, seed)
-    "cfg1": (1_000_000, 2_000_000, 20, 1),
-    "cfg0": (2_000, 5_000, 10, 0),
-    "mid": (100_000, 200_000, 20, 1),
+WORKLOADS = {  # name: (kind, description, generator)
+    "cfg1": ("single", "planted equality LP m=1,000,000 n=2,000,000 nnz/col=20 seed=1, 0<=x<=10",
+             lambda gen, a: gen.planted_lp(1_000_000, 2_000_000, 20, seed=1)),
+    "cfg0": ("single", "planted equality LP m=2,000 n=5,000 nnz/col=10 seed=0, 0<=x<=10",
+             lambda gen, a: gen.planted_lp(2_000, 5_000, 10, seed=0)),
+    "mid": ("single", "planted equality LP m=100,000 n=200,000 nnz/col=20 seed=1, 0<=x<=10",
+            lambda gen, a: gen.planted_lp(100_000, 200_000, 20, seed=1)),
+    "cfg2": ("single", "block-angular planning LP: 64 blocks of 5,000x8,000 (8 N(0,1)/col) + 200 linking rows "
+                       "(2% of each block's columns, +-1), planted, m=320,200 n=512,000 seed=2",
+             lambda gen, a: gen.block_angular_lp()),
+    "cfg3": ("batch", "independent planted LPs of uneven size, m~U[500,20000], n=round(m*U[1.5,3]), "
+                      "nnz/col~U{6..12} per LP, seed=3",
+             lambda gen, a: gen.uneven_batch(count=a.batch_count)),
 }
 
 
@@ -91,6 +103,174 @@ def cpu_baseline(md, iterations, sample_iters):
                        f"{iterations} iterations", ms_per_iter=per_iter * 1e3)
 
 
+def roofline_entry(b_row, b_col, row_avg, col_avg, label=None):
+    if col_avg >= row_avg:
+        dom, dom_bytes, dom_avg = "col (A^T z + primal step)", b_col, col_avg
+    else:
+        dom, dom_bytes, dom_avg = "row (A x + dual step)", b_row, row_avg
+    if label:
+        dom = label
+    achieved = dom_bytes / (dom_avg * 1e-3) / 1e9 if dom_avg > 0 else 0.0
+    return dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(dom_bytes),
+                avg_launch_ms=round(dom_avg, 5), row_kernel_avg_ms=round(row_avg, 5), col_kernel_avg_ms=round(col_avg, 5),
+                row_gbs=round(b_row / (row_avg * 1e-3) / 1e9, 1) if row_avg else 0,
+                col_gbs=round(b_col / (col_avg * 1e-3) / 1e9, 1) if col_avg else 0)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def solver_opts(a, mode_name, **extra):
+    o = dict(tol=a.tol, time_limit=a.time_limit, max_iter=a.max_iter, use_graph=0 if a.no_graph else 1,
+             verbose=int(a.verbose), progress_interval=a.progress, **mode_options(mode_name))
+    o.update(extra)
+    return o
+
+
+def rel_kkt_of(r):
+    return max(r["rel_primal"], r["rel_dual"], r["rel_gap"])
+
+
+def run_single(a, ffi, gen, desc, md):
+    model = ffi.Model.from_lp(md)
+    mode = a.mode
+
+    def timed(mode_name, steps, warmup, **extra):
+        t0 = time.perf_counter()
+        sess = ffi.Session(model, **solver_opts(a, mode_name, **extra))
+        setup = time.perf_counter() - t0
+        log(f"{mode_name}: session setup {setup:.1f}s")
+        for w in range(warmup):
+            r = sess.solve()
+            log(f"{mode_name} warmup {w}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s")
+        results = []
+        t0 = time.perf_counter()
+        for s_ in range(steps):
+            r = sess.solve()
+            results.append(r)
+            log(f"{mode_name} step {s_}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s")
+        wall = time.perf_counter() - t0
+        sess.close()
+        return wall, results, setup
+
+    wall, res, setup = timed(mode, a.steps, a.warmup)
+    last = res[-1]
+    value = a.steps / wall
+    # kernel roofline: a separate capped solve with HIP-event timing of every iteration launch (no graphs)
+    _, prof, _ = timed(mode, 1, 0, profile_kernels=1, max_iter=min(a.max_iter, a.profile_iters))
+    q = prof[-1]
+    s_val = 8 if MODES[mode]["precision_bits"] == 64 else 4
+    b_row, b_col = (halpern_kernel_bytes if MODES[mode].get("halpern") else kernel_bytes)(md.m, md.n, q["nnz"], s_val)
+    roofline = roofline_entry(b_row, b_col, q["row_kernel_ms"] / max(q["row_launches"], 1),
+                              q["col_kernel_ms"] / max(q["col_launches"], 1))
+    roofline["profiled_iterations"] = q["iterations"]
+    out = {
+        "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[mode], "data": "synthetic",
+        "config": {"workload": desc, "m": md.m, "n": md.n, "nnz": last["nnz"], "tol": a.tol, "mode": mode,
+                   "parallelism": "single GPU"},
+        "iterations": last["iterations"], "restarts": last["restarts"], "refinements": last["refinements"],
+        "status": last["status"], "rel_kkt": rel_kkt_of(last),
+        "iters_per_s": round(last["iterations"] / last["solve_s"], 1), "session_setup_s": round(setup, 3),
+        "device": last["device"], "roofline": roofline,
+    }
+    if md.planted_objective is not None:
+        p_star = md.planted_objective
+        out["objective_rel_error_vs_planted"] = abs(last["objective"] - p_star) / (1 + abs(p_star))
+    if mode != "fp64" and not a.no_fp64_compare:
+        # the plain FP64 cold start of the reference algorithm, capped at --fp64-time-limit
+        saved = a.time_limit
+        a.time_limit = a.fp64_time_limit
+        w64, r64, _ = timed("fp64", 1, 0)
+        a.time_limit = saved
+        q = r64[-1]
+        out["fp64_reference"] = dict(status=q["status"], iterations=q["iterations"], wall_s=round(w64, 3),
+                                     rel_kkt=rel_kkt_of(q), time_cap_s=a.fp64_time_limit)
+        if q["status"] == "near_optimal":
+            out["fp64_solves_per_s"] = round(1 / w64, 6)
+            out["speedup_vs_fp64"] = round(value * w64, 4)
+    if not a.no_cpu_baseline:
+        log("cpu baseline sample")
+        out["cpu_baseline"] = cpu_baseline(md, last["iterations"], a.cpu_sample_iters)
+    return out
+
+
+def run_batch(a, ffi, gen, desc, lps):
+    import numpy as np
+    if a.mode != "fast":
+        raise SystemExit("batch workloads run the fast path (--mode fast)")
+    models = [ffi.Model.from_lp(md) for md in lps]
+    t0 = time.perf_counter()
+    batch = ffi.Batch(models, cap=a.cap, **solver_opts(a, "fast"))
+    setup = time.perf_counter() - t0
+    info = batch.info()
+    log(f"batch of {len(lps)}: setup {setup:.1f}s, stage {info['cap']}, autotune ms {info['tune_ms']}")
+    for w in range(a.warmup):
+        batch.solve(points=False)
+    t0 = time.perf_counter()
+    for s_ in range(a.steps):
+        res = batch.solve(points=True)
+        log(f"batch step {s_}: {sum(r['status'] == 'near_optimal' for r in res)} of {len(res)} near_optimal")
+    wall = time.perf_counter() - t0
+    value = len(lps) * a.steps / wall
+    its = np.array([r["iterations"] for r in res])
+    # roofline: one batched iteration with every instance live (row launch + column launch), from the autotune
+    # timing of the chosen stage size (HIP events around 24 iterations on the batch stream)
+    b_it = 0
+    for md in lps:
+        r_, c_ = halpern_kernel_bytes(md.m, md.n, md.nnz, 4)
+        b_it += r_ + c_
+    ti = {1024: 0, 2048: 1, 4096: 2}.get(info["cap"])
+    it_ms = info["tune_ms"][ti] / 24 if ti is not None and info["tune_ms"][ti] > 0 else 0.0
+    achieved = b_it / (it_ms * 1e-3) / 1e9 if it_ms else 0.0
+    roofline = dict(bound="hbm", kernel="one batched iteration (row + column launch, all instances live)",
+                    achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=None, bytes_per_launch=int(b_it), avg_launch_ms=round(it_ms, 5))
+    errs = [abs(r["objective"] - md.planted_objective) / (1 + abs(md.planted_objective)) for r, md in zip(res, lps)]
+    out = {
+        "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES["fast"], "data": "synthetic",
+        "config": {"workload": f"{len(lps)} {desc}", "count": len(lps), "nnz_total": info["nnz_total"],
+                   "tol": a.tol, "mode": "fast (batched)", "stage": info["cap"], "parallelism": "single GPU"},
+        "status_counts": {s: sum(r["status"] == s for r in res) for s in sorted({r["status"] for r in res})},
+        "iterations": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max())},
+        "max_rel_kkt": max(rel_kkt_of(r) for r in res), "max_objective_rel_error_vs_planted": max(errs),
+        "session_setup_s": round(setup, 3), "device": res[0]["device"], "roofline": roofline,
+    }
+    if not a.no_fp64_compare and a.fp64_sample > 0:
+        # the plain FP64 cold start of the reference algorithm, one LP at a time, on the first LPs of the batch
+        k = min(a.fp64_sample, len(lps))
+        t0 = time.perf_counter()
+        ok = 0
+        for i in range(k):
+            r = ffi.solve(models[i], **solver_opts(a, "fp64", time_limit=a.fp64_time_limit))
+            ok += r["status"] == "near_optimal"
+            log(f"fp64 {lps[i].name}: {r['status']} it={r['iterations']} {r['solve_s']:.2f}s")
+        t64 = time.perf_counter() - t0
+        out["fp64_reference"] = dict(sample=f"first {k} LPs of the batch, one at a time", converged=ok, wall_s=round(t64, 3),
+                                     solves_per_s=round(k / t64, 4), time_cap_s=a.fp64_time_limit)
+        out["speedup_vs_fp64"] = round(value / (k / t64), 3)
+    if not a.no_cpu_baseline:
+        log("cpu baseline sample")
+        i_med = int(np.argsort([md.nnz for md in lps])[len(lps) // 2])
+        cb = cpu_baseline(lps[i_med], max(int(its[i_med]), 1), a.cpu_sample_iters)
+        # CPU time of the whole batch: every LP's iterations at the sample's per-nonzero iteration cost
+        per_nnz = cb["ms_per_iter"] * 1e-3 / lps[i_med].nnz
+        total = sum(int(r["iterations"]) * md.nnz * per_nnz for r, md in zip(res, lps))
+        cb["value"] = len(lps) / total
+        cb["sample"] += f"; whole batch = sum of each LP's GPU iteration count x its nonzeros at that per-nonzero cost"
+        out["cpu_baseline"] = cb
+    return out
+
+
+DTYPES = {"fp64": "f64", "fp32": "f32 (f64 reductions)", "fast": "f32 iterates + f64 refinement",
+          "halpern-fp64": "f64"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,9 +282,14 @@ def main():
     ap.add_argument("--time-limit", type=float, default=600.0)
     ap.add_argument("--max-iter", type=int, default=2_000_000_000,
                     help="iteration cap per solve (profiling runs; the reported value then is not a full solve)")
+    ap.add_argument("--profile-iters", type=int, default=4096,
+                    help="iterations of the separate HIP-event-timed solve behind the roofline figure")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--fp64-time-limit", type=float, default=120.0,
-                    help="time cap of the plain FP64 reference-algorithm solve reported beside the timed mode")
+                    help="time cap of each plain FP64 reference-algorithm solve reported beside the timed mode")
+    ap.add_argument("--fp64-sample", type=int, default=16, help="batch workloads: LPs solved one by one in FP64")
+    ap.add_argument("--batch-count", type=int, default=512, help="batch workloads: number of LPs")
+    ap.add_argument("--cap", type=int, default=0, help="batch row-block stage (0 = autotune)")
     ap.add_argument("--verbose", action="store_true", help="solver progress on stderr every 50 evaluations")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying HIP graphs (used under rocprofv3)")
@@ -117,93 +302,12 @@ def main():
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a design only "
               "(see the sharding section of the design notes)", file=sys.stderr)
         sys.exit(2)
-    mode = a.mode
-
     from taral_pdhg import ffi, gen
-    m, n, k, seed = WORKLOADS[a.workload]
+    kind, desc, make = WORKLOADS[a.workload]
     t = time.perf_counter()
-    md = gen.planted_lp(m, n, k, seed=seed)
-    model = ffi.Model.from_lp(md)
-    print(f"[bench] generated {m}x{n} nnz={md.nnz} in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
-
-    def timed(mode_name, steps, warmup, profile):
-        opts = dict(tol=a.tol, time_limit=a.time_limit, max_iter=a.max_iter, profile_kernels=int(profile),
-                    use_graph=0 if a.no_graph else 1, verbose=int(a.verbose), progress_interval=a.progress,
-                    **mode_options(mode_name))
-        t0 = time.perf_counter()
-        sess = ffi.Session(model, **opts)
-        setup = time.perf_counter() - t0
-        print(f"[bench] {mode_name}: session setup {setup:.1f}s", file=sys.stderr, flush=True)
-        for w in range(warmup):
-            r = sess.solve()
-            print(f"[bench] {mode_name} warmup {w}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s",
-                  file=sys.stderr, flush=True)
-        results = []
-        t0 = time.perf_counter()
-        for s in range(steps):
-            r = sess.solve()
-            results.append(r)
-            print(f"[bench] {mode_name} step {s}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s",
-                  file=sys.stderr, flush=True)
-        wall = time.perf_counter() - t0
-        sess.close()
-        return wall, results, setup
-
-    wall, res, setup = timed(mode, a.steps, a.warmup, profile=True)
-    last = res[-1]
-    value = a.steps / wall
-    s_val = 8 if MODES[mode]["precision_bits"] == 64 else 4
-    b_row, b_col = (halpern_kernel_bytes if MODES[mode].get("halpern") else kernel_bytes)(m, n, last["nnz"], s_val)
-    row_ms = sum(r["row_kernel_ms"] for r in res)
-    col_ms = sum(r["col_kernel_ms"] for r in res)
-    row_n = sum(r["row_launches"] for r in res)
-    col_n = sum(r["col_launches"] for r in res)
-    row_avg = row_ms / max(row_n, 1)
-    col_avg = col_ms / max(col_n, 1)
-    if col_ms >= row_ms:
-        dom, dom_bytes, dom_avg = "col (A^T z + primal step)", b_col, col_avg
-    else:
-        dom, dom_bytes, dom_avg = "row (A x + dual step)", b_row, row_avg
-    achieved = dom_bytes / (dom_avg * 1e-3) / 1e9 if dom_avg > 0 else 0.0
-    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=dom_bytes,
-                    avg_launch_ms=round(dom_avg, 5), row_kernel_avg_ms=round(row_avg, 5),
-                    col_kernel_avg_ms=round(col_avg, 5),
-                    row_gbs=round(b_row / (row_avg * 1e-3) / 1e9, 1) if row_avg else 0,
-                    col_gbs=round(b_col / (col_avg * 1e-3) / 1e9, 1) if col_avg else 0)
-    p_star = md.planted_objective
-    out = {
-        "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 (f64 reductions)", "fast": "f32 iterates + f64 refinement",
-                  "halpern-fp64": "f64"}[mode],
-        "data": "synthetic",
-        "config": {"workload": f"planted equality LP m={m} n={n} nnz/col={k} seed={seed}, 0<=x<=10, tol {a.tol:g}",
-                   "m": m, "n": n, "nnz": last["nnz"], "tol": a.tol, "mode": mode, "parallelism": "single GPU"},
-        "iterations": last["iterations"], "restarts": last["restarts"], "status": last["status"],
-        "rel_kkt": max(last["rel_primal"], last["rel_dual"], last["rel_gap"]),
-        "objective_rel_error_vs_planted": abs(last["objective"] - p_star) / (1 + abs(p_star)),
-        "iters_per_s": round(last["iterations"] / last["solve_s"], 1), "session_setup_s": round(setup, 3),
-        "device": last["device"], "roofline": roofline,
-    }
-    if mode != "fp64" and not a.no_fp64_compare:
-        # the plain FP64 cold start of the reference algorithm, capped: on the large planted LPs it needs far
-        # longer than the timed mode, so its status and KKT at the cap are reported instead of a ratio
-        saved = a.time_limit
-        a.time_limit = a.fp64_time_limit
-        w64, r64, _ = timed("fp64", 1, 0, profile=False)
-        a.time_limit = saved
-        q = r64[-1]
-        out["fp64_reference"] = dict(status=q["status"], iterations=q["iterations"], wall_s=round(w64, 3),
-                                     rel_kkt=max(q["rel_primal"], q["rel_dual"], q["rel_gap"]),
-                                     time_cap_s=a.fp64_time_limit)
-        if q["status"] == "near_optimal":
-            out["fp64_solves_per_s"] = round(1 / w64, 6)
-            out["speedup_vs_fp64"] = round(value * w64, 4)
-    if not a.no_cpu_baseline:
-        print("[bench] cpu baseline sample", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(md, last["iterations"], a.cpu_sample_iters)
+    data = make(gen, a)
+    log(f"generated {a.workload} in {time.perf_counter() - t:.1f}s")
+    out = run_single(a, ffi, gen, desc, data) if kind == "single" else run_batch(a, ffi, gen, desc, data)
     print(json.dumps(out), flush=True)
 
 

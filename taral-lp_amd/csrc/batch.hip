@@ -85,15 +85,18 @@ struct BEpiHCol {
 };
 
 // Row-block SpMV over the batch (the k_rowblocks scheme of hipcommon.hpp with a CAP-entry stage). blk holds
-// (instance, first row, end row) triples; rows are local to the instance.
+// (instance, first row, end row) triples; rows are local to the instance. With a list, workgroup b takes block
+// list[b] (nblk = list length), so a launch covers only the blocks of the instances still iterating.
 template <class T, int CAP, class Epi>
-__global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict__ blk, const int* __restrict__ pbase,
-                                                 const int* __restrict__ ptr, const int* __restrict__ idx,
-                                                 const T* __restrict__ val, const int* __restrict__ voff,
-                                                 const T* __restrict__ v, const uint8_t* __restrict__ on, Epi epi) {
+__global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict__ list, const int* __restrict__ blk,
+                                                 const int* __restrict__ pbase, const int* __restrict__ ptr,
+                                                 const int* __restrict__ idx, const T* __restrict__ val,
+                                                 const int* __restrict__ voff, const T* __restrict__ v,
+                                                 const uint8_t* __restrict__ on, Epi epi) {
     __shared__ T sh[CAP];
     __shared__ T red[kWG / 64];
-    const int b = xcd_block(int(blockIdx.x), nblk);
+    const int lb = xcd_block(int(blockIdx.x), nblk);
+    const int b = list ? list[lb] : lb;  // list: the live instances' blocks, in instance order
     const int inst = blk[3 * b], r0 = blk[3 * b + 1], r1 = blk[3 * b + 2];
     if (!on[inst]) return;  // the whole workgroup leaves together
     const int* P = ptr + pbase[inst];
@@ -354,6 +357,7 @@ public:
         if (hflags_) (void)hipHostFree(hflags_);
         if (hpar_) (void)hipHostFree(hpar_);
         if (hout_) (void)hipHostFree(hout_);
+        if (hlist_) (void)hipHostFree(hlist_);
         if (st_) (void)hipStreamDestroy(st_);
     }
 
@@ -472,6 +476,7 @@ public:
     void iterate(int steps, const std::vector<uint8_t>& on) override {
         if (steps <= 0) return;
         upload_act(on);
+        ensure_lists(on);
         if (o_.use_graph && !profile_ && steps == o_.eval_freq) {
             if (!graph_) {
                 hipGraph_t g = nullptr;
@@ -721,6 +726,27 @@ private:
             return t;
         };
         const std::vector<int32_t> ta = table(PA_), tt = table(PT_);
+        auto ranges = [&](const std::vector<int32_t>& t, std::vector<int32_t>& first, std::vector<int32_t>& count) {
+            first.assign(size_t(B_), 0), count.assign(size_t(B_), 0);
+            for (size_t k = 0; k < t.size() / 3; ++k) {
+                const int i = t[3 * k];
+                if (count[size_t(i)] == 0) first[size_t(i)] = int32_t(k);
+                count[size_t(i)]++;
+            }
+        };
+        ranges(ta, afirst_, acount_), ranges(tt, tfirst_, tcount_);
+        {  // live-block list buffers, grown to the largest table seen
+            const size_t na = std::max<size_t>(ta.size() / 3, 1), nt = std::max<size_t>(tt.size() / 3, 1);
+            if (na > list_cap_a_) alist_ = mem_.alloc<int>(na), list_cap_a_ = na;
+            if (nt > list_cap_t_) tlist_ = mem_.alloc<int>(nt), list_cap_t_ = nt;
+            if (na + nt > hlist_cap_) {
+                if (hlist_) HIP_OK(hipHostFree(hlist_));
+                hlist_ = nullptr;
+                HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hlist_), (na + nt) * sizeof(int), hipHostMallocDefault));
+                hlist_cap_ = na + nt;
+            }
+        }
+        list_inst_.clear(), nalist_ = ntlist_ = 0;
         auto it = tables_.find(cap);
         if (it == tables_.end()) {
             it = tables_.emplace(cap, std::make_pair(up_int(ta), up_int(tt))).first;
@@ -747,6 +773,7 @@ private:
         for (int cap : {1024, 2048, 4096}) {
             set_cap(cap);
             upload_act(all);
+            ensure_lists(all);
             launch_block(4);  // warm-up
             HIP_OK(hipEventRecord(e0, st_));
             for (int r = 0; r < 3; ++r) launch_block(8);
@@ -780,20 +807,53 @@ private:
     }
 
     template <class S, class U, class Epi>
-    void spmv_rows(const S& side, const U* v, const Epi& epi, const uint8_t* on) {
-        if (side.nblk == 0) return;
+    void spmv_rows(const S& side, const U* v, const Epi& epi, const uint8_t* on, const int* list = nullptr,
+                   int nlist = 0) {
+        const int nb = list ? nlist : side.nblk;
+        if (nb == 0) return;
         switch (cap_) {
             case 1024:
-                k_bblocks<U, 1024, Epi><<<side.nblk, kWG, 0, st_>>>(side.nblk, side.blk, side.pbase, side.ptr, side.idx,
-                                                                   side.val, side.voff, v, on, epi);
+                k_bblocks<U, 1024, Epi><<<nb, kWG, 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val,
+                                                            side.voff, v, on, epi);
                 break;
             case 4096:
-                k_bblocks<U, 4096, Epi><<<side.nblk, kWG, 0, st_>>>(side.nblk, side.blk, side.pbase, side.ptr, side.idx,
-                                                                   side.val, side.voff, v, on, epi);
+                k_bblocks<U, 4096, Epi><<<nb, kWG, 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val,
+                                                            side.voff, v, on, epi);
                 break;
             default:
-                k_bblocks<U, 2048, Epi><<<side.nblk, kWG, 0, st_>>>(side.nblk, side.blk, side.pbase, side.ptr, side.idx,
-                                                                   side.val, side.voff, v, on, epi);
+                k_bblocks<U, 2048, Epi><<<nb, kWG, 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val,
+                                                            side.voff, v, on, epi);
+        }
+    }
+    // Live-block lists of both sides for the instances in `on`: rebuilt when an instance in `on` is missing from
+    // the current lists or when the live blocks have dropped to half the lists' length (so at most ~log2 of the
+    // block count rebuilds per solve); a rebuild drops the captured graph, whose launches have the old lengths.
+    void ensure_lists(const std::vector<uint8_t>& on) {
+        int64_t live_a = 0;
+        bool missing = list_inst_.size() != size_t(B_);
+        for (int i = 0; i < B_ && !missing; ++i) missing = on[size_t(i)] && !list_inst_[size_t(i)];
+        for (int i = 0; i < B_; ++i)
+            if (on[size_t(i)]) live_a += acount_[size_t(i)];
+        if (!missing && 2 * live_a > int64_t(nalist_)) return;
+        std::vector<int32_t> la, lt;
+        list_inst_.assign(size_t(B_), 0);
+        for (int i = 0; i < B_; ++i) {
+            if (!on[size_t(i)]) continue;
+            list_inst_[size_t(i)] = 1;
+            for (int k = 0; k < acount_[size_t(i)]; ++k) la.push_back(afirst_[size_t(i)] + k);
+            for (int k = 0; k < tcount_[size_t(i)]; ++k) lt.push_back(tfirst_[size_t(i)] + k);
+        }
+        std::copy(la.begin(), la.end(), hlist_);
+        std::copy(lt.begin(), lt.end(), hlist_ + la.size());
+        if (!la.empty()) HIP_OK(hipMemcpyAsync(alist_, hlist_, la.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+        if (!lt.empty())
+            HIP_OK(hipMemcpyAsync(tlist_, hlist_ + la.size(), lt.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        nalist_ = int(la.size()), ntlist_ = int(lt.size());
+        ++list_builds_;
+        if (graph_) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
         }
     }
     BEpiHCol<T> col_epi(int s, int mode, bool store) {
@@ -818,14 +878,14 @@ private:
             if (profile_) {
                 const size_t e = pending_;
                 HIP_OK(hipEventRecord(event(e), st_));
-                spmv_rows(A_, w_, er, dact_);
+                spmv_rows(A_, w_, er, dact_, alist_, nalist_);
                 HIP_OK(hipEventRecord(event(e + 1), st_));
-                spmv_rows(AT_, z_, ec, dact_);
+                spmv_rows(AT_, z_, ec, dact_, tlist_, ntlist_);
                 HIP_OK(hipEventRecord(event(e + 2), st_));
                 pending_ += 3;
             } else {
-                spmv_rows(A_, w_, er, dact_);
-                spmv_rows(AT_, z_, ec, dact_);
+                spmv_rows(A_, w_, er, dact_, alist_, nalist_);
+                spmv_rows(AT_, z_, ec, dact_, tlist_, ntlist_);
             }
         }
         k_bset_counters<<<(B_ + kWG - 1) / kWG, kWG, 0, st_>>>(B_, dact_, kc_, steps, 0);
@@ -856,6 +916,12 @@ private:
     std::vector<int32_t> rtab_, ctab_, rfirst_, rcount_, cfirst_, ccount_;
     int nrseg_ = 0, ncseg_ = 0, cap_ = kCap;
     std::map<int, std::pair<int*, int*>> tables_;
+    std::vector<int32_t> afirst_, acount_, tfirst_, tcount_;
+    std::vector<uint8_t> list_inst_;
+    int *alist_ = nullptr, *tlist_ = nullptr, *hlist_ = nullptr;
+    size_t list_cap_a_ = 0, list_cap_t_ = 0, hlist_cap_ = 0;
+    int nalist_ = 0, ntlist_ = 0;
+    int64_t list_builds_ = 0;
     std::vector<float> tune_ms_;
     Side A_, AT_;
     Side64 A64_, AT64_;

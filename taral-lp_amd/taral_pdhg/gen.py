@@ -295,3 +295,66 @@ class ScenarioFamily:
         b = self.base
         return LPModel(m=b.m, n=b.n, colptr=b.colptr, rowidx=b.rowidx, val=b.val, cost=c, col_lo=l, col_up=u,
                        row_lo=rl, row_up=ru, name=name)
+
+
+def _planted_from_coo(m, n, rows, cols, vals, rng, lo=0.0, up=10.0, name="LP") -> LPModel:
+    """The planted construction (x*: half zeros, half U(0,1); b = A x*; y* ~ N(0,1); s_j ~ U(0,1) where x*_j = 0;
+    c = A^T y* + s; equality rows; lo <= x <= up) on a matrix given as COO entries."""
+    order = np.lexsort((rows, cols))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    colptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(cols, minlength=n), out=colptr[1:])
+    xstar = np.zeros(n)
+    nz = rng.permutation(n)[n // 2:]
+    xstar[nz] = rng.random(nz.size)
+    b = np.bincount(rows, weights=vals * xstar[cols], minlength=m)
+    ystar = rng.standard_normal(m)
+    s = np.where(xstar == 0, rng.random(n), 0.0)
+    c = np.bincount(cols, weights=vals * ystar[rows], minlength=n) + s
+    return LPModel(m=m, n=n, colptr=colptr, rowidx=rows.astype(np.int32), val=vals, cost=c,
+                   col_lo=np.full(n, lo), col_up=np.full(n, up), row_lo=b.copy(), row_up=b.copy(), name=name,
+                   planted_objective=float(np.dot(c, xstar)), x_star=xstar, y_star=ystar)
+
+
+def block_angular_lp(blocks: int = 64, rows_per_block: int = 5000, cols_per_block: int = 8000,
+                     nnz_per_col: int = 8, linking_rows: int = 200, link_frac: float = 0.02,
+                     seed: int = 2) -> LPModel:
+    """Block-angular planning LP: ``blocks`` diagonal blocks (rows_per_block x cols_per_block, ``nnz_per_col``
+    N(0,1) entries per column at distinct rows of its block) plus ``linking_rows`` rows that each touch a random
+    ``link_frac`` of the columns of every block with coefficients +-1; right-hand side and costs planted as in
+    planted_lp. Rows: blocks * rows_per_block block rows, then the linking rows."""
+    rng = np.random.default_rng(seed)
+    n = blocks * cols_per_block
+    m = blocks * rows_per_block + linking_rows
+    local = _distinct_rows(rng, rows_per_block, n, nnz_per_col)              # per column, rows within its block
+    block_of = np.arange(n) // cols_per_block
+    r_blk = (local + (block_of * rows_per_block)[:, None]).reshape(-1)
+    c_blk = np.repeat(np.arange(n), nnz_per_col)
+    v_blk = rng.standard_normal(r_blk.size)
+    per_block = max(1, int(round(link_frac * cols_per_block)))
+    r_l, c_l = [], []
+    for k in range(linking_rows):
+        for p in range(blocks):
+            sel = rng.choice(cols_per_block, per_block, replace=False) + p * cols_per_block
+            c_l.append(sel)
+            r_l.append(np.full(per_block, blocks * rows_per_block + k))
+    r_l, c_l = np.concatenate(r_l), np.concatenate(c_l)
+    v_l = np.where(rng.random(r_l.size) < 0.5, -1.0, 1.0)
+    rows = np.concatenate([r_blk, r_l]).astype(np.int64)
+    cols = np.concatenate([c_blk, c_l]).astype(np.int64)
+    vals = np.concatenate([v_blk, v_l])
+    return _planted_from_coo(m, n, rows, cols, vals, rng, name=f"BLOCKANGULAR_{blocks}x{rows_per_block}x{cols_per_block}")
+
+
+def uneven_batch(count: int = 512, m_lo: int = 500, m_hi: int = 20000, ratio=(1.5, 3.0), nnz=(6, 12),
+                 seed: int = 3) -> list:
+    """``count`` planted LPs of uneven size: m ~ U[m_lo, m_hi], n = round(m * U[ratio]), one nonzeros-per-column
+    count per LP drawn from U{nnz[0]..nnz[1]}; LP k uses seed (seed * 100003 + k)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        m = int(rng.integers(m_lo, m_hi + 1))
+        n = int(round(m * rng.uniform(*ratio)))
+        kk = int(rng.integers(nnz[0], nnz[1] + 1))
+        out.append(planted_lp(m, n, kk, seed=seed * 100003 + k, name=f"BATCH{k}_{m}x{n}x{kk}"))
+    return out
