@@ -6,6 +6,9 @@ Workloads (--workload):
                   column at uniform-random rows, N(0,1) values, 0 <= x <= 10, seed 1.
   cfg2            the block-angular planning LP (64 blocks of 5,000 x 8,000 plus 200 linking rows), planted.
   cfg3            a batch of 512 independent planted LPs of uneven size, solved together by the batched fast path.
+  cfg4            a scenario family of one planted LP (base 5,000 x 10,000 by default, --scenario-m): the learned
+                  warm start is trained on cold solves of 2,048 scenarios, then one step predicts and solves the
+                  512 held-out scenarios (batches of 512 sharing the matrix), with the cold-start fallback.
   cfg0, mid       the 2,000 x 5,000 and 100,000 x 200,000 planted LPs.
 Every solve runs to 1e-6 relative KKT (original-space primal residual, dual residual and gap, the reference's
 stopping test). A "step" is one full solve from a cold start of problems already resident in HBM (power iteration,
@@ -50,6 +53,9 @@ WORKLOADS = {  # name: (kind, description, generator)
     "cfg3": ("batch", "independent planted LPs of uneven size, m~U[500,20000], n=round(m*U[1.5,3]), "
                       "nnz/col~U{6..12} per LP, seed=3",
              lambda gen, a: gen.uneven_batch(count=a.batch_count)),
+    "cfg4": ("scenarios", "scenario family of one planted LP (6 nnz/col, seed 4): 32 factors theta~N(0,0.1^2), "
+                          "each rescaling a fixed 0.15% of b and of c by exp(theta)",
+             lambda gen, a: gen.ScenarioFamily(gen.planted_lp(a.scenario_m, 2 * a.scenario_m, 6, seed=4), seed=4)),
 }
 
 
@@ -267,6 +273,99 @@ def run_batch(a, ffi, gen, desc, lps):
     return out
 
 
+def run_scenarios(a, ffi, gen, desc, fam):
+    import numpy as np
+    from taral_pdhg import learned
+    if a.mode != "fast":
+        raise SystemExit("scenario workloads run the fast path (--mode fast)")
+    th_tr, th_te = fam.thetas(a.train_scenarios, seed=1), fam.thetas(a.test_scenarios, seed=2)
+    bs = min(a.scenario_batch, a.test_scenarios)
+    # training data: cold-start PDHG (the batched fast path) on the training scenarios
+    rtr, s_tr = learned.solve_family(fam, th_tr, batch_size=a.scenario_batch,
+                                     log=lambda s_: log(f"training data: {s_}"), progress_interval=a.progress)
+    X = np.array([r["x"] for r in rtr]); Y = np.array([r["y"] for r in rtr])
+    W = np.array([r["primal_weight"] for r in rtr])
+    wm = learned.WarmStartModel(fam, seed=1)
+    fit = wm.fit(th_tr, X, Y, W, epochs=a.epochs)
+    log(f"network trained on {len(th_tr)} scenarios: {fit}")
+    solver = learned.ScenarioSolver(fam, bs, progress_interval=a.progress, warm_fallback=1)
+    info = solver.batch.info()
+
+    def held_out(warm):
+        res, t0 = [], time.perf_counter()
+        for i0 in range(0, len(th_te), bs):
+            th = th_te[i0:i0 + bs]
+            if warm:
+                p = wm.predict(th)
+                res += solver.solve(th, p[0], p[1], p[2])
+            else:
+                res += solver.solve(th)
+        return res, time.perf_counter() - t0
+
+    cold, s_cold = held_out(False)
+    log(f"held-out cold: {len(th_te) / s_cold:.2f} solves/s")
+    for w in range(a.warmup):
+        held_out(True)
+    t0 = time.perf_counter()
+    for s_ in range(a.steps):
+        res, _ = held_out(True)
+        log(f"held-out learned step {s_}: {sum(r['status'] == 'near_optimal' for r in res)} near_optimal")
+    wall = time.perf_counter() - t0
+    value = len(th_te) * a.steps / wall
+    it_w = np.array([r["iterations"] for r in res]); it_c = np.array([r["iterations"] for r in cold])
+    b_it = 0
+    for _ in range(bs):
+        r_, c_ = halpern_kernel_bytes(fam.base.m, fam.base.n, fam.base.nnz, 4)
+        b_it += r_ + c_
+    ti = {1024: 0, 2048: 1, 4096: 2}.get(info["cap"])
+    it_ms = info["tune_ms"][ti] / 24 if ti is not None and info["tune_ms"][ti] > 0 else 0.0
+    achieved = b_it / (it_ms * 1e-3) / 1e9 if it_ms else 0.0
+    out = {
+        "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES["fast"] + "; network bf16 (MFMA) / f32",
+        "data": "synthetic",
+        "config": {"workload": f"{desc}; base m={fam.base.m} n={fam.base.n}; {len(th_tr)} training and "
+                               f"{len(th_te)} held-out scenarios", "tol": a.tol,
+                   "mode": "learned warm start + fast path, batches of %d" % bs, "parallelism": "single GPU"},
+        "learned": {"training_solves_s": round(s_tr, 3), "train_s": round(fit["train_s"], 3),
+                    "bases": fit["bases"], "explained_variance": round(fit["explained_variance"], 4),
+                    "warm_used": int(sum(r["warm_start_used"] for r in res)),
+                    "median_warm_kkt": float(np.median([r["warm_kkt"] for r in res])),
+                    "median_cold_kkt": float(np.median([r["cold_kkt"] for r in res]))},
+        "cold_batched_solves_per_s": round(len(th_te) / s_cold, 4),
+        "learned_gain_vs_cold_batched": round(value * s_cold / len(th_te), 3),
+        "iterations": {"learned_mean": float(it_w.mean()), "cold_mean": float(it_c.mean()),
+                       "learned_max": int(it_w.max()), "cold_max": int(it_c.max())},
+        "status_counts": {s2: sum(r["status"] == s2 for r in res) for s2 in sorted({r["status"] for r in res})},
+        "max_rel_kkt": max(rel_kkt_of(r) for r in res),
+        "roofline": dict(bound="hbm", kernel=f"one batched iteration of {bs} scenarios (row + column launch, all "
+                                             "live; the shared matrix is stored once, so these are the bytes of "
+                                             "unshared storage and partly cache hits)",
+                         achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(b_it),
+                         avg_launch_ms=round(it_ms, 5)),
+    }
+    if not a.no_fp64_compare and a.fp64_sample > 0:
+        k = min(a.fp64_sample, len(th_te))
+        t0 = time.perf_counter()
+        ok = 0
+        for i in range(k):
+            r = ffi.solve(ffi.Model.from_lp(fam.model(th_te[i])), **solver_opts(a, "fp64", time_limit=a.fp64_time_limit))
+            ok += r["status"] == "near_optimal"
+            log(f"fp64 scenario {i}: {r['status']} it={r['iterations']} {r['solve_s']:.2f}s")
+        t64 = time.perf_counter() - t0
+        out["fp64_reference"] = dict(sample=f"first {k} held-out scenarios, one at a time, cold start", converged=ok,
+                                     wall_s=round(t64, 3), solves_per_s=round(k / t64, 4),
+                                     time_cap_s=a.fp64_time_limit)
+        out["speedup_vs_fp64"] = round(value / (k / t64), 3)
+    if not a.no_cpu_baseline:
+        log("cpu baseline sample")
+        out["cpu_baseline"] = cpu_baseline(fam.model(th_te[0]), max(int(it_c.mean()), 1), a.cpu_sample_iters)
+        out["cpu_baseline"]["sample"] += " (the held-out scenarios' mean cold iteration count)"
+    return out
+
+
 DTYPES = {"fp64": "f64", "fp32": "f32 (f64 reductions)", "fast": "f32 iterates + f64 refinement",
           "halpern-fp64": "f64"}
 
@@ -290,6 +389,13 @@ def main():
     ap.add_argument("--fp64-sample", type=int, default=16, help="batch workloads: LPs solved one by one in FP64")
     ap.add_argument("--batch-count", type=int, default=512, help="batch workloads: number of LPs")
     ap.add_argument("--cap", type=int, default=0, help="batch row-block stage (0 = autotune)")
+    ap.add_argument("--scenario-m", type=int, default=5000,
+                    help="scenario workloads: rows of the base LP (n = 2m); the configuration's 50,000 needs ~1.5M "
+                         "iterations per scenario, so the default is a tenth of it")
+    ap.add_argument("--train-scenarios", type=int, default=2048)
+    ap.add_argument("--test-scenarios", type=int, default=512)
+    ap.add_argument("--scenario-batch", type=int, default=512)
+    ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--verbose", action="store_true", help="solver progress on stderr every 50 evaluations")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying HIP graphs (used under rocprofv3)")
@@ -307,7 +413,8 @@ def main():
     t = time.perf_counter()
     data = make(gen, a)
     log(f"generated {a.workload} in {time.perf_counter() - t:.1f}s")
-    out = run_single(a, ffi, gen, desc, data) if kind == "single" else run_batch(a, ffi, gen, desc, data)
+    run = {"single": run_single, "batch": run_batch, "scenarios": run_scenarios}[kind]
+    out = run(a, ffi, gen, desc, data)
     print(json.dumps(out), flush=True)
 
 
