@@ -92,20 +92,25 @@ def mode_options(mode):
 
 
 def cpu_baseline(md, iterations, sample_iters):
-    """Oracle FP64 (one thread): time `sample_iters` PDHG iterations of the same problem, scale to one solve
-    of `iterations` iterations (power-iteration steps counted as iterations too)."""
+    """Oracle FP64 (one thread): time PDHG iterations of the same problem (at least `sample_iters`, doubled until
+    the sample takes half a second), scale to one solve of `iterations` iterations."""
     from oracle import oracle
     o = oracle.OracleLP(md)
     t0 = time.perf_counter()
     o.block(0.5, 0.5, 0)  # start: two products
     t_start = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    o.block(0.5, 0.5, sample_iters)
-    t = time.perf_counter() - t0 - t_start
-    per_iter = max(t, 1e-9) / sample_iters
+    k = max(1, sample_iters)
+    while True:
+        t0 = time.perf_counter()
+        o.block(0.5, 0.5, k)
+        t = time.perf_counter() - t0 - t_start
+        if t >= 0.5 or k >= 1 << 16:
+            break
+        k *= 2
+    per_iter = max(t, 1e-9) / k
     return dict(value=1.0 / (per_iter * iterations), unit="solves/s", cores=1, kind="port",
-                sample=f"CPU oracle (oracle/pdhg_oracle.c, FP64, 1 thread): {sample_iters} PDHG iterations of the "
-                       f"same LP took {t:.3f}s ({per_iter * 1e3:.1f} ms/iter); scaled to the GPU solve's "
+                sample=f"CPU oracle (oracle/pdhg_oracle.c, FP64, 1 thread): {k} PDHG iterations of the "
+                       f"same LP took {t:.3f}s ({per_iter * 1e3:.3f} ms/iter); scaled to the GPU solve's "
                        f"{iterations} iterations", ms_per_iter=per_iter * 1e3)
 
 
@@ -122,6 +127,21 @@ def roofline_entry(b_row, b_col, row_avg, col_avg, label=None):
                 avg_launch_ms=round(dom_avg, 5), row_kernel_avg_ms=round(row_avg, 5), col_kernel_avg_ms=round(col_avg, 5),
                 row_gbs=round(b_row / (row_avg * 1e-3) / 1e9, 1) if row_avg else 0,
                 col_gbs=round(b_col / (col_avg * 1e-3) / 1e9, 1) if col_avg else 0)
+
+
+def measured_traffic(workload, mode, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload (copy-kernel calibrated), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"pdhg_{workload}_{mode}_traffic.md")
+    side = "col" if kernel.startswith("col") else "row"
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == side:
+                return int(parts[2])
+    return None
 
 
 def log(msg):
@@ -172,6 +192,7 @@ def run_single(a, ffi, gen, desc, md):
     roofline = roofline_entry(b_row, b_col, q["row_kernel_ms"] / max(q["row_launches"], 1),
                               q["col_kernel_ms"] / max(q["col_launches"], 1))
     roofline["profiled_iterations"] = q["iterations"]
+    roofline["traffic"] = measured_traffic(a.workload, mode, roofline["kernel"])
     out = {
         "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
@@ -403,7 +424,23 @@ def main():
                     help="seconds between the solver's progress lines on stderr (0 = none)")
     ap.add_argument("--cpu-sample-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="only time a plain device-to-device copy of this many GiB (3 times) and exit: the "
+                         "calibration kernel for the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
+    if a.copy_probe_gib > 0:
+        import torch
+        nbytes = int(a.copy_probe_gib * (1 << 30))
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 3
+        print(json.dumps({"copy_bytes": nbytes, "ms": dt * 1e3, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a design only "
               "(see the sharding section of the design notes)", file=sys.stderr)
