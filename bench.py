@@ -151,6 +151,8 @@ def log(msg):
 def solver_opts(a, mode_name, **extra):
     o = dict(tol=a.tol, time_limit=a.time_limit, max_iter=a.max_iter, use_graph=0 if a.no_graph else 1,
              verbose=int(a.verbose), progress_interval=a.progress, **mode_options(mode_name))
+    if a.omega_rule is not None:
+        o["omega_rule"] = a.omega_rule
     o.update(extra)
     return o
 
@@ -301,15 +303,17 @@ def run_scenarios(a, ffi, gen, desc, fam):
         raise SystemExit("scenario workloads run the fast path (--mode fast)")
     th_tr, th_te = fam.thetas(a.train_scenarios, seed=1), fam.thetas(a.test_scenarios, seed=2)
     bs = min(a.scenario_batch, a.test_scenarios)
+    extra = {} if a.omega_rule is None else {"omega_rule": a.omega_rule}
     # training data: cold-start PDHG (the batched fast path) on the training scenarios
     rtr, s_tr = learned.solve_family(fam, th_tr, batch_size=a.scenario_batch,
-                                     log=lambda s_: log(f"training data: {s_}"), progress_interval=a.progress)
+                                     log=lambda s_: log(f"training data: {s_}"), progress_interval=a.progress,
+                                     **extra)
     X = np.array([r["x"] for r in rtr]); Y = np.array([r["y"] for r in rtr])
     W = np.array([r["primal_weight"] for r in rtr])
     wm = learned.WarmStartModel(fam, seed=1)
     fit = wm.fit(th_tr, X, Y, W, epochs=a.epochs)
     log(f"network trained on {len(th_tr)} scenarios: {fit}")
-    solver = learned.ScenarioSolver(fam, bs, progress_interval=a.progress, warm_fallback=1)
+    solver = learned.ScenarioSolver(fam, bs, progress_interval=a.progress, warm_fallback=1, **extra)
     info = solver.batch.info()
 
     def held_out(warm):
@@ -410,6 +414,9 @@ def main():
     ap.add_argument("--fp64-sample", type=int, default=16, help="batch workloads: LPs solved one by one in FP64")
     ap.add_argument("--batch-count", type=int, default=512, help="batch workloads: number of LPs")
     ap.add_argument("--cap", type=int, default=0, help="batch row-block stage (0 = autotune)")
+    ap.add_argument("--omega-rule", type=int, default=None,
+                    help="fast path: primal-weight guard (0 reference update, 1 bounded, 2 balance, 3 both; "
+                         "default: the library's, 3)")
     ap.add_argument("--scenario-m", type=int, default=5000,
                     help="scenario workloads: rows of the base LP (n = 2m); the configuration's 50,000 needs ~1.5M "
                          "iterations per scenario, so the default is a tenth of it")

@@ -169,6 +169,10 @@ typedef struct {
                              (no reference counterpart: the reference always cold-starts, gpu/pdhg.cu:450-458) */
     int32_t warm_fallback; /* 1: keep a warm start only when its relative KKT is below the cold start's */
     double progress_interval; /* > 0: one progress line on stderr every this many seconds of solving */
+    int32_t omega_rule; /* fast path and batches: primal-weight guard after each restart. 0: the reference's
+                           smoothed update only; 1: kept within two decades of the starting weight; 2: residual
+                           balancing (no move against a 10x primal/gap imbalance); 3 (default): both. The
+                           reference scheme (halpern = 0) always uses the reference update */
 } tpdhg_options;
 void tpdhg_options_default(tpdhg_options* o);
 

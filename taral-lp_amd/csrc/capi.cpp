@@ -79,6 +79,7 @@ tpdhg::Options to_options(const tpdhg_options* o) {
     r.verbose = o->verbose != 0, r.profile_kernels = o->profile_kernels != 0;
     r.halpern = o->halpern != 0, r.rho = o->reflection;
     r.omega0 = o->primal_weight, r.warm_fallback = o->warm_fallback != 0, r.progress_s = o->progress_interval;
+    r.omega_rule = o->omega_rule;
     return r;
 }
 }  // namespace
@@ -316,6 +317,7 @@ void tpdhg_options_default(tpdhg_options* o) {
     o->verbose = 0, o->profile_kernels = 0;
     o->halpern = d.halpern ? 1 : 0, o->reflection = d.rho;
     o->primal_weight = d.omega0, o->warm_fallback = d.warm_fallback ? 1 : 0, o->progress_interval = d.progress_s;
+    o->omega_rule = d.omega_rule;
 }
 
 namespace {
@@ -347,6 +349,7 @@ int check_options(const tpdhg::Options& o) {
     if (o.refine && !o.halpern) return fail(TPDHG_ERR_ARG, "refine runs on the Halpern path: set halpern = 1");
     if (o.halpern && !(o.rho >= 1 && o.rho < 2)) return fail(TPDHG_ERR_ARG, "reflection must be in [1, 2)");
     if (!(o.omega0 >= 0) || !std::isfinite(o.omega0)) return fail(TPDHG_ERR_ARG, "primal_weight must be finite and >= 0");
+    if (o.omega_rule < 0 || o.omega_rule > 3) return fail(TPDHG_ERR_ARG, "omega_rule must be 0, 1, 2 or 3");
     return TPDHG_OK;
 }
 }  // namespace

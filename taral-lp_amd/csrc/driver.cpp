@@ -26,6 +26,24 @@ bool progress_due(const Options& o, Clock::time_point& last) {
     return true;
 }
 
+// Primal weight after a restart. Rule 0 is the reference's smoothed update exp(0.5 ln(dz/dx) + 0.5 ln w)
+// (gpu/pdhg.cu:839-840), skipped when either distance is below 1e-10. The fast path's optional guards:
+//   1  keep the weight within two decades of the solve's starting weight w_start;
+//   2  residual balancing: when the relative primal residual exceeds ten times the larger of the dual residual
+//      and the gap, the weight may not fall (a small weight makes the primal step large and the dual step small,
+//      which leaves the primal residual behind); in the opposite case it may not rise;
+//   3  both.
+double next_omega(double w, double dx, double dz, double p, double dg, double w_start, int rule) {
+    double nw = w;
+    if (dx > 1e-10 && dz > 1e-10) nw = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w));
+    if (rule == 2 || rule == 3) {
+        if (p > 10 * dg) nw = std::max(nw, w);
+        else if (dg > 10 * p) nw = std::min(nw, w);
+    }
+    if (rule == 1 || rule == 3) nw = std::min(std::max(nw, w_start * 1e-2), w_start * 1e2);
+    return nw;
+}
+
 // Original-space relative residuals of one evaluation (gpu/pdhg.cu:791-796).
 Rel rel_of(const double* k, const Scaled& P, double row_rel) {
     const double po = k[POBJ], du = k[DROW] + k[DCOL];
@@ -161,6 +179,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
     cs = std::sqrt(cs), bs = std::sqrt(bs);
     double w = cs > 1e-10 && bs > 1e-10 ? cs / bs : 1.0;
     if (omega0 > 0) w = omega0;
+    const double w_start = w;
     b.set_base(x0, z0);
     auto wkkt = [&](const double* k) {
         const double g = k[POBJ] - (k[DROW] + k[DCOL]);
@@ -225,7 +244,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
         k_prev = kc;
         if (restart) {
             const double dx = std::sqrt(kk[DX2]), dz = std::sqrt(kk[DZ2]);
-            if (dx > 1e-10 && dz > 1e-10) w = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w));
+            w = next_omega(w, dx, dz, rc.p, std::max(rc.d, rc.g), w_start, o.omega_rule);
             if (o.refine && r.refinements < o.max_refinements) {
                 recentre();
                 r.refinements++;
@@ -383,6 +402,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
         w[size_t(i)] = cs > 1e-10 && bs > 1e-10 ? cs / bs : 1.0;
         if (size_t(i) < omega0.size() && omega0[size_t(i)] > 0) w[size_t(i)] = omega0[size_t(i)];
     }
+    const std::vector<double> w_start = w;
     auto push_steps = [&]() {
         for (int i = 0; i < B; ++i) tau[size_t(i)] = eta[size_t(i)] / w[size_t(i)], sig[size_t(i)] = eta[size_t(i)] * w[size_t(i)];
         b.set_steps(tau, sig);
@@ -449,8 +469,8 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
             k_prev[size_t(i)] = kc;
             if (restart) {
                 const double dx = std::sqrt(k[DX2]), dz = std::sqrt(k[DZ2]);
-                if (dx > 1e-10 && dz > 1e-10)
-                    w[size_t(i)] = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w[size_t(i)]));
+                w[size_t(i)] = next_omega(w[size_t(i)], dx, dz, rc.p, std::max(rc.d, rc.g), w_start[size_t(i)],
+                                          o.omega_rule);
                 if (o.refine && q.refinements < o.max_refinements) rs_move[size_t(i)] = 1, any_move = true, q.refinements++;
                 rs_any[size_t(i)] = 1, any_rs = true, q.restarts++;
             }

@@ -48,6 +48,8 @@ struct Options {  // defaults of gpu/pdhg.cu:744-750 plus this build's additions
     double omega0 = 0;         // initial primal weight of a warm-started solve (0: ||c||/||b|| as for a cold start)
     bool warm_fallback = false;  // cold-start instead when the warm point's relative KKT is not below the cold one's
     double progress_s = 0;       // > 0: a progress line on stderr every progress_s seconds of solving
+    int omega_rule = 3;          // fast path and batches: primal-weight guard (driver.cpp:next_omega); the reference
+                                 // scheme always uses the reference update
 };
 
 struct Outcome {

@@ -71,7 +71,7 @@ class Options(C.Structure):
                 ("refine", C.c_int32), ("inner_tol", C.c_double), ("max_refinements", C.c_int32),
                 ("use_graph", C.c_int32), ("verbose", C.c_int32), ("profile_kernels", C.c_int32),
                 ("halpern", C.c_int32), ("reflection", C.c_double), ("primal_weight", C.c_double),
-                ("warm_fallback", C.c_int32), ("progress_interval", C.c_double)]
+                ("warm_fallback", C.c_int32), ("progress_interval", C.c_double), ("omega_rule", C.c_int32)]
 
 
 class Result(C.Structure):
