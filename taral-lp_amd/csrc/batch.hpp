@@ -4,7 +4,8 @@
 // evaluation; instances with an identical scaled matrix share one copy of it on the device (scenario families).
 //
 // Arithmetic per instance is the FastDevice's exactly: the same row blocks, the same epilogues and the same
-// reduction partition, so a batch reproduces the single-instance fast path bit for bit (non-slab layouts).
+// reduction partition, so a batch reproduces the single-instance fast path bit for bit (non-slab layouts, rows of
+// at most 2,048 nonzeros: a longer row is summed by one workgroup here and in 2,048-nonzero chunks there).
 // No reference counterpart: the reference solves one LP per process (gpu/pdhg.cu:913-935).
 #pragma once
 #include <cstdint>

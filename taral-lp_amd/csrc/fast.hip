@@ -354,6 +354,8 @@ private:
             d.rows = a.rows, d.nblk = a.nblk, d.maxblk = a.maxblk, d.slab = a.slab;
             d.ptr = a.ptr, d.idx = a.idx, d.blk = a.blk, d.snblk = a.snblk, d.sboff = a.sboff;
             d.val = reinterpret_cast<double*>(a.val), d.part = reinterpret_cast<double*>(a.part);
+            d.nch = a.nch, d.nlr = a.nlr, d.chunk = a.chunk, d.lrow = a.lrow;
+            d.lpart = reinterpret_cast<double*>(a.lpart);
             return d;
         } else {
             return share_structure<double>(mem_, st_, a, L, h.val.data());

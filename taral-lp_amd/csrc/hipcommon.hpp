@@ -45,26 +45,34 @@ struct EpiStore {  // y = M v
     __device__ void operator()(int r, T s) const { y[r] = s; }
 };
 
+// Row-block SpMV. Workgroups [0, nblk) take the row blocks; a row longer than kCap nonzeros forms a block of its
+// own, which is skipped here: workgroups [nblk, nblk + nch) each take one kCap-nonzero chunk of such a row
+// (chunk c: row, first and end nonzero in chunk[3c..3c+2]) and write its sum to lpart[c]; k_long_rows then adds
+// each long row's chunk sums in chunk order and applies the epilogue (deterministic, no atomics).
 template <class T, class Epi>
 __global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restrict__ blk, const int* __restrict__ ptr,
                                                    const int* __restrict__ idx, const T* __restrict__ val,
-                                                   const T* __restrict__ v, Epi epi) {
+                                                   const T* __restrict__ v, Epi epi, const int* __restrict__ chunk,
+                                                   T* __restrict__ lpart) {
     __shared__ T sh[kCap];
     __shared__ T red[kWG / 64];
-    const int b = xcd_block(int(blockIdx.x), nblk);
     const int tid = int(threadIdx.x);
-    const int r0 = blk[b], r1 = blk[b + 1];
-    const int k0 = ptr[r0], k1 = ptr[r1];
-    epi.load();
-    if (k1 - k0 > kCap) {  // a single long row: strided partial sums and a workgroup reduction
+    if (int(blockIdx.x) >= nblk) {  // a chunk of a long row
+        const int c = int(blockIdx.x) - nblk;
+        const int k0 = chunk[3 * c + 1], k1 = chunk[3 * c + 2];
         T s = 0;
         for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * v[idx[k]];
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
-        if (tid == 0) epi(r0, (red[0] + red[1]) + (red[2] + red[3]));
+        if (tid == 0) lpart[c] = (red[0] + red[1]) + (red[2] + red[3]);
         return;
     }
+    const int b = xcd_block(int(blockIdx.x), nblk);
+    const int r0 = blk[b], r1 = blk[b + 1];
+    const int k0 = ptr[r0], k1 = ptr[r1];
+    if (k1 - k0 > kCap) return;  // a long row: its chunks and k_long_rows handle it
+    epi.load();
 #pragma unroll
     for (int u = 0; u < kCap / kWG; ++u) {
         const int k = k0 + u * kWG + tid;
@@ -82,6 +90,19 @@ __global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restri
     }
     for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (row < nr && lane == 0) epi(r0 + row, s);
+}
+
+// Long rows: row lrow[3i], chunks [lrow[3i+1], lrow[3i+1] + lrow[3i+2]) of lpart, summed in order.
+template <class T, class Epi>
+__global__ __launch_bounds__(kWG) void k_long_rows(int nlr, const int* __restrict__ lrow, const T* __restrict__ lpart,
+                                                   Epi epi) {
+    epi.load();
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < nlr; i += int(gridDim.x * blockDim.x)) {
+        const int c0 = lrow[3 * i + 1], nc = lrow[3 * i + 2];
+        T s = lpart[c0];
+        for (int c = 1; c < nc; ++c) s += lpart[c0 + c];
+        epi(lrow[3 * i], s);
+    }
 }
 
 // Slab layout, first kernel: workgroup b handles row block b / 8 of slab b % 8 (slab = XCD) and writes the
@@ -272,12 +293,16 @@ struct DevMat {
     int *snblk = nullptr, *sboff = nullptr;              // slab: block count and block-list offset per slab
     T* val = nullptr;
     T* part = nullptr;  // slab: kSlabs * rows partial sums
+    int nch = 0, nlr = 0;                                // plain: chunks of long rows, long rows
+    int *chunk = nullptr, *lrow = nullptr;
+    T* lpart = nullptr;                                  // plain: one partial sum per chunk
 };
 
 struct HostLayout {
     bool slab = false;
     int nblk = 0, maxblk = 0;
     std::vector<int32_t> ptr, idx, blk, snblk, sboff;
+    std::vector<int32_t> chunk, lrow;  // plain: long-row chunks (row, k0, k1) and long rows (row, first chunk, count)
     std::vector<int64_t> perm;  // slab layout: position k holds the CSR entry perm[k]
 };
 
@@ -290,6 +315,14 @@ HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* id
         L.idx.assign(idx, idx + nnz);
         L.blk = row_blocks(rows, ptr);
         L.nblk = int(L.blk.size()) - 1;
+        for (int i = 0; i < rows; ++i) {
+            if (ptr[i + 1] - ptr[i] <= kCap) continue;
+            L.lrow.push_back(i), L.lrow.push_back(int32_t(L.chunk.size() / 3));
+            int nc = 0;
+            for (int32_t k = ptr[i]; k < ptr[i + 1]; k += kCap, ++nc)
+                L.chunk.push_back(i), L.chunk.push_back(k), L.chunk.push_back(std::min(k + kCap, ptr[i + 1]));
+            L.lrow.push_back(nc);
+        }
         return L;
     }
     auto slab_of = [&](int32_t c) { return int(int64_t(c) * kSlabs / cols); };
@@ -350,6 +383,8 @@ DevMat<T> upload_layout(Mem& mem, hipStream_t st, int rows, const HostLayout& L,
         return p;
     };
     d.ptr = up(L.ptr), d.idx = up(L.idx), d.blk = up(L.blk);
+    d.nch = int(L.chunk.size() / 3), d.nlr = int(L.lrow.size() / 3);
+    if (d.nch) d.chunk = up(L.chunk), d.lrow = up(L.lrow), d.lpart = mem.alloc<T>(size_t(d.nch));
     if (L.slab) {
         d.snblk = up(L.snblk), d.sboff = up(L.sboff);
         d.part = mem.alloc<T>(size_t(kSlabs) * size_t(std::max(rows, 1)));
@@ -365,6 +400,8 @@ DevMat<U> share_structure(Mem& mem, hipStream_t st, const DevMat<T>& a, const Ho
     DevMat<U> d;
     d.rows = a.rows, d.nblk = a.nblk, d.maxblk = a.maxblk, d.slab = a.slab;
     d.ptr = a.ptr, d.idx = a.idx, d.blk = a.blk, d.snblk = a.snblk, d.sboff = a.sboff;
+    d.nch = a.nch, d.nlr = a.nlr, d.chunk = a.chunk, d.lrow = a.lrow;
+    if (a.nch) d.lpart = mem.alloc<U>(size_t(a.nch));
     d.val = upload_values<U>(mem, st, L, val, int64_t(L.idx.size()));
     if (a.slab) d.part = mem.alloc<U>(size_t(kSlabs) * size_t(std::max(a.rows, 1)));
     return d;
@@ -387,7 +424,8 @@ void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st)
         return;
     }
     if (a.nblk == 0) return;
-    k_rowblocks<T, Epi><<<a.nblk, kWG, 0, st>>>(a.nblk, a.blk, a.ptr, a.idx, a.val, v, epi);
+    k_rowblocks<T, Epi><<<a.nblk + a.nch, kWG, 0, st>>>(a.nblk, a.blk, a.ptr, a.idx, a.val, v, epi, a.chunk, a.lpart);
+    if (a.nlr) k_long_rows<T, Epi><<<(a.nlr + kWG - 1) / kWG, kWG, 0, st>>>(a.nlr, a.lrow, a.lpart, epi);
 }
 
 std::string device_name() {
