@@ -222,6 +222,22 @@ void tpdhg_session_free(tpdhg_session* s);
 int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const tpdhg_result* result, char* buf,
                       int32_t len);
 
+/* ------------------------------------------------------------------ original-model recheck */
+/* The checks of gpu/vector_check.py:46-74 and benchmarks/orig_check.py:63-88 on the model as read (host only):
+ * objective c'x + const, the largest row violation relative to (1 + |violated bound|), the largest bound
+ * violation, and, with row multipliers y, the weak-duality bound
+ *   LB(y) = sum_j min_{x_j in [l_j,u_j]} (c - A'y)_j x_j + sum_i min_{r_i in [lo_i,up_i]} y_i r_i + const
+ * for the sign of y (+ or -) with the smaller dual infeasibility (a maximisation is checked as min -c'x and its
+ * bound reported in its own sense: an upper bound). */
+typedef struct {
+    double objective, row_viol_rel, bound_viol;
+    int32_t dual_checked;      /* y was given */
+    int32_t dual_bound_finite; /* no wrong-sign term on an infinite bound: dual_bound is a rigorous bound */
+    int32_t dual_sign;         /* +1 or -1 */
+    double dual_bound, dual_obj_loose, dual_infeas_rel, dual_gap_rel;
+} tpdhg_recheck_result;
+int tpdhg_recheck(const tpdhg_model* model, const double* x, const double* y, tpdhg_recheck_result* out);
+
 /* ------------------------------------------------------------------ batches (no reference counterpart) */
 /* Many LPs solved together by the fast path on one device: block-diagonal packing, one row launch and one column
  * launch per iteration for all live instances, per-instance step sizes, restarts and stopping, one HIP graph per

@@ -10,6 +10,7 @@
 #include "../../include/taral_pdhg.h"
 #include "driver.hpp"
 #include "mlp.hpp"
+#include "recheck.hpp"
 
 struct tpdhg_model {
     tpdhg::LP lp;
@@ -430,6 +431,18 @@ int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const 
         r.out.warm_used = res->warm_start_used != 0;
         std::string s = tpdhg::report_json(r, model->lp, to_options(opt));
         std::snprintf(buf, size_t(len), "%s", s.c_str());
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_recheck(const tpdhg_model* model, const double* x, const double* y, tpdhg_recheck_result* out) {
+    if (!model || !x || !out) return fail(TPDHG_ERR_ARG, "null argument");
+    return guarded([&]() -> int {
+        const tpdhg::Recheck c = tpdhg::recheck(model->lp, x, y);
+        out->objective = c.objective, out->row_viol_rel = c.row_viol_rel, out->bound_viol = c.bound_viol;
+        out->dual_checked = c.dual_checked, out->dual_bound_finite = c.dual_bound_finite, out->dual_sign = c.dual_sign;
+        out->dual_bound = c.dual_bound, out->dual_obj_loose = c.dual_obj_loose;
+        out->dual_infeas_rel = c.dual_infeas_rel, out->dual_gap_rel = c.dual_gap_rel;
         return TPDHG_OK;
     });
 }

@@ -7,8 +7,10 @@
 // Additional flags: --fast (the fast path: FP32 reflected Halpern PDHG iterates inside FP64 iterative
 // refinement = --fp32 --halpern --refine), --halpern, --reflection R, --refine,
 // --max-refinements N, --profile-kernels, --warm-sol FILE / --warm-dual FILE (start point in the .sol/.dual
-// formats this program writes).
+// formats this program writes), --recheck (original-model recheck: chk_* keys in the JSON), --batch LIST (many
+// models in one batched solve).
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +21,7 @@
 #include <vector>
 
 #include "driver.hpp"
+#include "recheck.hpp"
 
 using namespace tpdhg;
 using Clock = std::chrono::steady_clock;
@@ -43,9 +46,100 @@ static bool read_named(const char* path, const std::vector<std::string>& names, 
     return true;
 }
 
+// Appends the original-model recheck (gpu/bench_pdhg.py's chk_* keys) to a JSON object, and prints it.
+void add_recheck(std::string& js, const LP& lp, const Report& r, bool print) {
+    const Recheck c = recheck(lp, r.x.data(), r.y.data());
+    char num[64];
+    auto fmt = [&](double v) {
+        std::snprintf(num, sizeof num, "%.17g", v);
+        return std::string(std::isfinite(v) ? num : "null");
+    };
+    if (print)
+        std::printf("recheck objective %.12g row_viol_rel %.2e bound_viol %.2e dual_bound %s dual_gap_rel %.2e "
+                    "dual_infeas_rel %.2e\n",
+                    c.objective, c.row_viol_rel, c.bound_viol, c.dual_bound_finite ? fmt(c.dual_bound).c_str() : "none",
+                    c.dual_gap_rel, c.dual_infeas_rel);
+    const std::string extra = ", \"chk_objective\": " + fmt(c.objective) + ", \"chk_row_viol_rel\": " +
+                              fmt(c.row_viol_rel) + ", \"chk_bound_viol\": " + fmt(c.bound_viol) +
+                              ", \"chk_dual_bound\": " + (c.dual_bound_finite ? fmt(c.dual_bound) : "null") +
+                              ", \"chk_dual_obj_loose\": " + fmt(c.dual_obj_loose) + ", \"chk_dual_infeas_rel\": " +
+                              fmt(c.dual_infeas_rel) + ", \"chk_dual_gap_rel\": " + fmt(c.dual_gap_rel) +
+                              ", \"chk_dual_sign\": " + std::to_string(c.dual_sign);
+    const size_t brace = js.rfind('}');
+    if (brace != std::string::npos) js.insert(brace, extra);
+}
+
+void status_line(const Report& r) {
+    std::printf("status %s objective %.12g iterations %ld restarts %d rel_p %.2e rel_d %.2e rel_gap %.2e setup %.3fs "
+                "solve %.3fs total %.3fs device %s\n",
+                r.out.status.c_str(), r.objective, long(r.out.iterations), r.out.restarts, r.out.rel_primal,
+                r.out.rel_dual, r.out.rel_gap, r.setup_s, r.solve_s, r.total_s, r.device_name.c_str());
+}
+
+// --batch LIST: every MPS path listed (one per line; blank lines and lines starting with # skipped) is read, gated
+// and solved in one batch by the fast path; one "case PATH" + status line per LP on stdout and, with --json, one
+// JSON object per LP and line (the single-solve keys plus "case"). Exit codes as for one model.
+int run_batch_list(const char* list, Options o, const char* json, bool do_recheck) {
+    std::ifstream in(list);
+    if (!in) {
+        std::fprintf(stderr, "cannot read %s\n", list);
+        return 2;
+    }
+    std::vector<std::string> paths;
+    for (std::string line; std::getline(in, line);) {
+        while (!line.empty() && (line.back() == '\r' || line.back() == ' ' || line.back() == '\t')) line.pop_back();
+        size_t a = line.find_first_not_of(" \t");
+        if (a == std::string::npos || line[a] == '#') continue;
+        paths.push_back(line.substr(a));
+    }
+    if (paths.empty()) {
+        std::fprintf(stderr, "%s lists no models\n", list);
+        return 2;
+    }
+    std::vector<LP> lps(paths.size());
+    for (size_t k = 0; k < paths.size(); ++k) {
+        try {
+            lps[k] = flatten(read_mps(paths[k]));
+        } catch (const ParseError& e) {
+            std::printf("status parse_error: %s: %s\n", paths[k].c_str(), e.what());
+            return 1;
+        }
+        if (const char* why = input_error(lps[k])) {
+            std::printf("status unsupported_model\n");
+            std::fflush(stdout);
+            std::fprintf(stderr, "%s: %s\n", paths[k].c_str(), why);
+            return 2;
+        }
+    }
+    o.halpern = true;  // batches run the fast path
+    std::vector<Report> reps;
+    try {
+        std::vector<const LP*> ptrs;
+        for (const LP& lp : lps) ptrs.push_back(&lp);
+        BatchSession bs(ptrs, o);
+        reps = bs.solve();
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "HIP error: %s\n", e.what());
+        return 1;
+    }
+    std::FILE* f = json ? std::fopen(json, "w") : nullptr;
+    for (size_t k = 0; k < paths.size(); ++k) {
+        std::printf("case %s\n", paths[k].c_str());
+        status_line(reps[k]);
+        std::string js = report_json(reps[k], lps[k], o);
+        if (do_recheck) add_recheck(js, lps[k], reps[k], true);
+        while (!js.empty() && (js.back() == '\n' || js.back() == ' ')) js.pop_back();
+        if (!js.empty() && js.front() == '{') js.insert(1, "\"case\": \"" + paths[k] + "\", ");
+        if (f) std::fprintf(f, "%s\n", js.c_str());
+    }
+    if (f) std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     const char *model = nullptr, *json = nullptr, *sol = nullptr, *dual = nullptr;
-    const char *warm_sol = nullptr, *warm_dual = nullptr;
+    const char *warm_sol = nullptr, *warm_dual = nullptr, *batch_list = nullptr;
+    bool do_recheck = false;
     std::string device = "gpu";
     int threads = 0;  // accepted for compatibility; the GPU path has no host threads to size
     Options o;
@@ -78,16 +172,20 @@ int main(int argc, char** argv) {
         else if (a == "--max-refinements") o.max_refinements = std::atoi(next());
         else if (a == "--profile-kernels") o.profile_kernels = true;
         else if (a == "--warm-sol") warm_sol = next();
+        else if (a == "--recheck") do_recheck = true;
+        else if (a == "--batch") batch_list = next();
         else if (a == "--warm-dual") warm_dual = next();
         else model = argv[i];
     }
+    if (batch_list && !model && device == "gpu") return run_batch_list(batch_list, o, json, do_recheck);
     if (!model || (device != "gpu" && device != "cpu")) {
         std::fprintf(stderr,
                      "usage: pdhg MODEL.mps --device gpu|cpu [--threads N] [--tol 1e-4] [--time-limit S] "
                      "[--max-iter N] [--json OUT.json] [--sol OUT.sol] [--dual OUT.dual] [--fp32] [--eval-freq 64] "
                      "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--refine] "
                      "[--max-refinements 8] "
-                     "[--profile-kernels] [--warm-sol X.sol] [--warm-dual X.dual]\n");
+                     "[--profile-kernels] [--warm-sol X.sol] [--warm-dual X.dual] [--recheck]\n"
+                     "       pdhg --batch LIST.txt [--json OUT.jsonl] [solver flags]\n");
         return 2;
     }
     auto t_start = Clock::now();
@@ -128,13 +226,12 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "HIP error: %s\n", e.what());
         return 1;
     }
-    std::printf("status %s objective %.12g iterations %ld restarts %d rel_p %.2e rel_d %.2e rel_gap %.2e setup %.3fs "
-                "solve %.3fs total %.3fs device %s\n",
-                r.out.status.c_str(), r.objective, long(r.out.iterations), r.out.restarts, r.out.rel_primal,
-                r.out.rel_dual, r.out.rel_gap, r.setup_s, r.solve_s, r.total_s, r.device_name.c_str());
+    status_line(r);
+    std::string js = report_json(r, lp, o);
+    if (do_recheck) add_recheck(js, lp, r, true);
     if (json)
         if (std::FILE* f = std::fopen(json, "w")) {
-            std::fputs(report_json(r, lp, o).c_str(), f);
+            std::fputs(js.c_str(), f);
             std::fclose(f);
         }
     if (sol)

@@ -35,6 +35,7 @@ EXPORTS = [
     "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
     "tpdhg_session_set_vectors", "tpdhg_session_set_primal_weight",
     "tpdhg_result_json", "tpdhg_spmv_bench",
+    "tpdhg_recheck",
     "tpdhg_batch_create", "tpdhg_batch_get_info", "tpdhg_batch_set_vectors", "tpdhg_batch_solve", "tpdhg_batch_free",
     "tpdhg_mlp_create", "tpdhg_mlp_param_count", "tpdhg_mlp_set_params", "tpdhg_mlp_get_params",
     "tpdhg_mlp_forward", "tpdhg_mlp_loss_grad", "tpdhg_mlp_train", "tpdhg_mlp_destroy",
@@ -86,6 +87,13 @@ class Result(C.Structure):
                 ("nnz", C.c_int64), ("row_kernel_ms", C.c_double), ("col_kernel_ms", C.c_double),
                 ("row_launches", C.c_int64), ("col_launches", C.c_int64), ("device", C.c_char * 64),
                 ("warm_start_used", C.c_int32), ("warm_kkt", C.c_double), ("cold_kkt", C.c_double)]
+
+
+class RecheckResult(C.Structure):
+    _fields_ = [("objective", C.c_double), ("row_viol_rel", C.c_double), ("bound_viol", C.c_double),
+                ("dual_checked", C.c_int32), ("dual_bound_finite", C.c_int32), ("dual_sign", C.c_int32),
+                ("dual_bound", C.c_double), ("dual_obj_loose", C.c_double), ("dual_infeas_rel", C.c_double),
+                ("dual_gap_rel", C.c_double)]
 
 
 class BatchInfo(C.Structure):
@@ -155,6 +163,7 @@ def lib():
         L.tpdhg_session_set_primal_weight.argtypes = [C.c_void_p, C.c_double]
         L.tpdhg_result_json.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result), C.c_char_p, C.c_int32]
         L.tpdhg_spmv_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d]
+        L.tpdhg_recheck.argtypes = [C.c_void_p, _d, _d, C.POINTER(RecheckResult)]
         _pp = C.POINTER(C.c_void_p)
         L.tpdhg_batch_create.argtypes = [_pp, C.c_int32, C.POINTER(Options), C.c_int32, C.POINTER(C.c_void_p)]
         L.tpdhg_batch_get_info.argtypes = [C.c_void_p, C.POINTER(BatchInfo)]
@@ -481,6 +490,18 @@ def solve(model: Model, x0=None, y0=None, **opts) -> dict:
     x0, y0 = _f64(x0), _f64(y0)
     _check(lib().tpdhg_solve(model._h, C.byref(o), _p(x0), _p(y0), C.byref(r), _p(x), _p(y)))
     return _result_dict(model, o, r, x, y, i.m, i.n)
+
+
+def recheck(model: "Model", x, y=None) -> dict:
+    """Original-model recheck of a point (objective, violations, weak-duality bound from y); host only."""
+    x = _f64(x)
+    y = _f64(y)
+    r = RecheckResult()
+    _check(lib().tpdhg_recheck(model._h, _p(x), _p(y), C.byref(r)))
+    out = {k: getattr(r, k) for k, _ in RecheckResult._fields_}
+    if not r.dual_bound_finite:
+        out["dual_bound"] = None
+    return out
 
 
 def _ptr_array(arrays):

@@ -135,3 +135,20 @@ def test_batch_create_validates_before_the_device():
         with pytest.raises(ffi.TpdhgError) as e:
             ffi.Batch([md, md])
         assert e.value.code == ffi.ERR_NO_DEVICE
+
+
+def test_cli_batch_list_gates_before_the_device(tmp_path):
+    lst = tmp_path / "list.txt"
+    r = subprocess.run([ffi.CLI_PATH, "--batch", str(tmp_path / "missing.txt")], capture_output=True, text=True)
+    assert r.returncode == 2
+    lst.write_text("# comment\n\n" + os.path.join(MPS, "gate_continuous_lp.mps") + "\n"
+                   + os.path.join(MPS, "gate_intorg.mps") + "\n")
+    r = subprocess.run([ffi.CLI_PATH, "--batch", str(lst)], capture_output=True, text=True)
+    assert r.returncode == 2 and r.stdout.strip() == "status unsupported_model" and "gate_intorg" in r.stderr
+    lst.write_text(os.path.join(MPS, "err_bad_number.mps") + "\n")
+    r = subprocess.run([ffi.CLI_PATH, "--batch", str(lst)], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stdout.startswith("status parse_error:")
+    if ffi.device_count() == 0:
+        lst.write_text(os.path.join(MPS, "gate_continuous_lp.mps") + "\n")
+        r = subprocess.run([ffi.CLI_PATH, "--batch", str(lst)], capture_output=True, text=True)
+        assert r.returncode == 1 and "no HIP device" in r.stderr
