@@ -167,8 +167,8 @@ public:
         name_ = device_name();
         HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         {
-            const HostLayout LA = make_layout(m_, n_, P.A.ptr.data(), P.A.idx.data());
-            const HostLayout LT = make_layout(n_, m_, P.AT.ptr.data(), P.AT.idx.data());
+            const HostLayout LA = make_layout(m_, n_, P.A.ptr.data(), P.A.idx.data(), int(sizeof(T)));
+            const HostLayout LT = make_layout(n_, m_, P.AT.ptr.data(), P.AT.idx.data(), int(sizeof(T)));
             A_ = upload_layout<T>(mem_, st_, m_, LA, P.A.val.data());
             AT_ = upload_layout<T>(mem_, st_, n_, LT, P.AT.val.data());
             A64_ = values64(A_, LA, P.A);
@@ -356,6 +356,7 @@ private:
             d.val = reinterpret_cast<double*>(a.val), d.part = reinterpret_cast<double*>(a.part);
             d.nch = a.nch, d.nlr = a.nlr, d.chunk = a.chunk, d.lrow = a.lrow;
             d.lpart = reinterpret_cast<double*>(a.lpart);
+            d.sorted = a.sorted, d.cbits = a.cbits, d.sb = a.sb, d.sstart = a.sstart;
             return d;
         } else {
             return share_structure<double>(mem_, st_, a, L, h.val.data());

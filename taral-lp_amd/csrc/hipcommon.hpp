@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -102,6 +103,103 @@ __global__ __launch_bounds__(kWG) void k_long_rows(int nlr, const int* __restric
         T s = lpart[c0];
         for (int c = 1; c < nc; ++c) s += lpart[c0 + c];
         epi(lrow[3 * i], s);
+    }
+}
+
+// Sorted layout. Inside each row block (at most SB nonzeros, rows of at most kCap) the nonzeros are stored in
+// ascending column order, each with its row-order position in the block packed above the column:
+// pk = (pos << cbits) | col. A wave's 64 lanes then gather from nearby columns, so several lanes share a cache
+// line and the L2 sees fewer requests than nonzeros; each product goes to LDS at its row-order position and every
+// row is summed there in CSR order by one thread. Rows longer than kCap are chunked as in k_rowblocks.
+template <class T, int SB, class Epi>
+__global__ __launch_bounds__(kWG) void k_sorted(int nblk, const int* __restrict__ blk, const int* __restrict__ ptr,
+                                                const uint32_t* __restrict__ pk, const T* __restrict__ val,
+                                                const T* __restrict__ v, int cbits, Epi epi,
+                                                const int* __restrict__ chunk, T* __restrict__ lpart) {
+    __shared__ T sh[SB];
+    __shared__ T red[kWG / 64];
+    const int tid = int(threadIdx.x);
+    const uint32_t mask = (1u << cbits) - 1u;
+    if (int(blockIdx.x) >= nblk) {  // a chunk of a long row
+        const int c = int(blockIdx.x) - nblk;
+        const int k0 = chunk[3 * c + 1], k1 = chunk[3 * c + 2];
+        T s = 0;
+        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * v[pk[k] & mask];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) lpart[c] = (red[0] + red[1]) + (red[2] + red[3]);
+        return;
+    }
+    const int b = xcd_block(int(blockIdx.x), nblk);
+    const int r0 = blk[b], r1 = blk[b + 1];
+    const int k0 = ptr[r0], k1 = ptr[r1];
+    if (k1 - k0 > kCap && r1 - r0 == 1) return;  // a long row: its chunks and k_long_rows handle it
+    epi.load();
+    for (int base = k0; base < k1; base += 8 * kWG) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = base + u * kWG + tid;
+            if (k < k1) {
+                const uint32_t p = pk[k];
+                sh[p >> cbits] = val[k] * v[p & mask];
+            }
+        }
+    }
+    __syncthreads();
+    for (int r = r0 + tid; r < r1; r += kWG) {
+        const int a = ptr[r] - k0, e = ptr[r + 1] - k0;
+        T s = 0;
+        for (int j = a; j < e; ++j) s += sh[j];
+        epi(r, s);
+    }
+}
+
+// Sorted slab layout: workgroup b takes row block b / 8 of slab b % 8 (columns local to the slab, gathered from
+// v + sstart[slab]) and writes the slab's partial row sums part[slab * rows + row]; same staging as k_sorted.
+template <class T, int SB>
+__global__ __launch_bounds__(kWG) void k_sorted_slab(int rows, const int* __restrict__ snblk,
+                                                     const int* __restrict__ sboff, const int* __restrict__ blk_all,
+                                                     const int* __restrict__ ptr_all, const uint32_t* __restrict__ pk,
+                                                     const T* __restrict__ val, const T* __restrict__ v,
+                                                     const int* __restrict__ sstart, int cbits, T* __restrict__ part) {
+    __shared__ T sh[SB];
+    __shared__ T red[kWG / 64];
+    const int q = int(blockIdx.x) & (kSlabsK - 1), lb = int(blockIdx.x) / kSlabsK;
+    if (lb >= snblk[q]) return;
+    const int* blk = blk_all + sboff[q];
+    const int* ptr = ptr_all + size_t(q) * (size_t(rows) + 1);
+    const T* vq = v + sstart[q];
+    T* y = part + size_t(q) * size_t(rows);
+    const int tid = int(threadIdx.x);
+    const uint32_t mask = (1u << cbits) - 1u;
+    const int r0 = blk[lb], r1 = blk[lb + 1];
+    const int k0 = ptr[r0], k1 = ptr[r1];
+    if (k1 - k0 > kCap && r1 - r0 == 1) {  // a long row of this slab: strided sums and a workgroup reduction
+        T s = 0;
+        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * vq[pk[k] & mask];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) y[r0] = (red[0] + red[1]) + (red[2] + red[3]);
+        return;
+    }
+    for (int base = k0; base < k1; base += 8 * kWG) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = base + u * kWG + tid;
+            if (k < k1) {
+                const uint32_t p = pk[k];
+                sh[p >> cbits] = val[k] * vq[p & mask];
+            }
+        }
+    }
+    __syncthreads();
+    for (int r = r0 + tid; r < r1; r += kWG) {
+        const int a = ptr[r] - k0, e = ptr[r + 1] - k0;
+        T s = 0;
+        for (int j = a; j < e; ++j) s += sh[j];
+        y[r] = s;
     }
 }
 
@@ -258,6 +356,25 @@ std::vector<int32_t> row_blocks(int rows, const int32_t* ptr) {
     return b;
 }
 
+// Row blocks of the sorted layout: maximal runs of rows with at most sb nonzeros, no row longer than kCap inside
+// a run; a row longer than kCap forms a block of its own.
+std::vector<int32_t> sorted_blocks(int rows, const int32_t* ptr, int sb) {
+    std::vector<int32_t> b;
+    b.push_back(0);
+    int r0 = 0;
+    while (r0 < rows) {
+        int r = r0;
+        if (int64_t(ptr[r0 + 1]) - ptr[r0] > kCap) {
+            r = r0 + 1;
+        } else {
+            while (r < rows && int64_t(ptr[r + 1]) - ptr[r] <= kCap && int64_t(ptr[r + 1]) - ptr[r0] <= sb) ++r;
+        }
+        b.push_back(r);
+        r0 = r;
+    }
+    return b;
+}
+
 int grid_for(int k) { return std::max(1, std::min(2048, (k + kWG - 1) / kWG)); }
 int red_blocks(int k) { return k > 0 ? std::max(1, std::min(kMaxRedBlocks, (k + kWG - 1) / kWG)) : 0; }
 
@@ -296,6 +413,9 @@ struct DevMat {
     int nch = 0, nlr = 0;                                // plain: chunks of long rows, long rows
     int *chunk = nullptr, *lrow = nullptr;
     T* lpart = nullptr;                                  // plain: one partial sum per chunk
+    bool sorted = false;                                 // idx holds packed (position, column) words
+    int cbits = 0, sb = 0;
+    int* sstart = nullptr;                               // sorted slab: first column of each slab
 };
 
 struct HostLayout {
@@ -303,17 +423,73 @@ struct HostLayout {
     int nblk = 0, maxblk = 0;
     std::vector<int32_t> ptr, idx, blk, snblk, sboff;
     std::vector<int32_t> chunk, lrow;  // plain: long-row chunks (row, k0, k1) and long rows (row, first chunk, count)
-    std::vector<int64_t> perm;  // slab layout: position k holds the CSR entry perm[k]
+    std::vector<int64_t> perm;  // slab or sorted layout: position k holds the CSR entry perm[k]
+    bool sorted = false;
+    int cbits = 0, sb = 0;  // sorted: column bits of the packed words, nonzeros per block (power of two)
+    std::vector<int32_t> sstart;
 };
 
-HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* idx) {
+// Largest sorted blocks (32 KB of FP32 products in LDS; measured at config 1: 8,192 beats 4,096 and 2,048 and
+// 16,384 loses to occupancy). Plain layout: 13 position bits for columns < 2^19; slab: 14 bits, slab columns < 2^18.
+constexpr int kSortPlain = 8192;
+constexpr int kSortSlab = 8192;
+constexpr int kSortMinBlocks = 1024;  // the block size shrinks (down to kCap) until there are this many blocks
+
+int sorted_block_size(int64_t nnz, int max_sb, int value_bytes) {
+    int sb = value_bytes > 4 ? max_sb / 2 : max_sb;  // same LDS bytes for FP64 values
+    if (const char* e = std::getenv("TPDHG_SORT_MAX_BLOCK")) {  // tuning: cap on the block size
+        const int cap = std::atoi(e);
+        while (sb > kCap && sb > cap) sb /= 2;
+    }
+    while (sb > kCap && nnz / sb < kSortMinBlocks) sb /= 2;
+    return sb;
+}
+
+// Whether the sorted layout is used (default on; TPDHG_LAYOUT=unsorted selects the earlier kernels).
+bool sorted_layout_enabled() {
+    const char* e = std::getenv("TPDHG_LAYOUT");
+    return !(e && std::string(e) == "unsorted");
+}
+
+// Sorts the entries of every block of a row range by column, writing packed words and the permutation (entries
+// [base + k0, base + k1) of idx_in map to the same range of pk / perm; positions are relative to the block).
+void sort_blocks(const std::vector<int32_t>& blk, const int32_t* ptr, const int32_t* cols_local, int64_t perm_base_in,
+                 const int64_t* perm_in, int cbits, std::vector<int32_t>& pk, std::vector<int64_t>& perm) {
+    std::vector<int32_t> order;
+    for (size_t b = 0; b + 1 < blk.size(); ++b) {
+        const int32_t k0 = ptr[blk[b]], k1 = ptr[blk[b + 1]];
+        const int32_t len = k1 - k0;
+        order.resize(size_t(len));
+        for (int32_t i = 0; i < len; ++i) order[size_t(i)] = i;
+        const bool single_long = blk[b + 1] - blk[b] == 1 && len > kCap;
+        if (!single_long)
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int32_t a, int32_t c) { return cols_local[k0 + a] < cols_local[k0 + c]; });
+        for (int32_t i = 0; i < len; ++i) {
+            const int32_t src = k0 + order[size_t(i)];
+            const uint32_t pos = single_long ? 0u : uint32_t(order[size_t(i)]);
+            pk[size_t(k0 + i)] = int32_t((pos << cbits) | uint32_t(cols_local[src]));
+            perm[size_t(k0 + i)] = perm_in ? perm_in[src] : perm_base_in + src;
+        }
+    }
+}
+
+HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* idx, int value_bytes = 4) {
     HostLayout L;
     const int64_t nnz = ptr[rows];
     L.slab = cols >= kSlabMinCols && rows > 0;
+    const bool want_sorted = sorted_layout_enabled();
     if (!L.slab) {
         L.ptr.assign(ptr, ptr + rows + 1);
-        L.idx.assign(idx, idx + nnz);
-        L.blk = row_blocks(rows, ptr);
+        if (want_sorted && cols <= (1 << 19)) {
+            L.sorted = true, L.cbits = 19, L.sb = sorted_block_size(nnz, kSortPlain, value_bytes);
+            L.blk = sorted_blocks(rows, ptr, L.sb);
+            L.idx.resize(size_t(nnz)), L.perm.resize(size_t(nnz));
+            sort_blocks(L.blk, ptr, idx, 0, nullptr, L.cbits, L.idx, L.perm);
+        } else {
+            L.idx.assign(idx, idx + nnz);
+            L.blk = row_blocks(rows, ptr);
+        }
         L.nblk = int(L.blk.size()) - 1;
         for (int i = 0; i < rows; ++i) {
             if (ptr[i + 1] - ptr[i] <= kCap) continue;
@@ -346,15 +522,31 @@ HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* id
         }
     L.ptr.resize(size_t(kSlabs) * (size_t(rows) + 1));
     L.snblk.resize(kSlabs), L.sboff.resize(kSlabs);
+    // slab q holds the columns c with c * kSlabs / cols == q, i.e. [ceil(q * cols / kSlabs), ...)
+    L.sstart.resize(kSlabs + 1);
+    for (int q = 0; q <= kSlabs; ++q) L.sstart[size_t(q)] = int32_t((int64_t(q) * cols + kSlabs - 1) / kSlabs);
+    int64_t width = 0;
+    for (int q = 0; q < kSlabs; ++q) width = std::max<int64_t>(width, L.sstart[size_t(q) + 1] - L.sstart[size_t(q)]);
+    L.sorted = want_sorted && width <= (1 << 18);
+    if (L.sorted) L.cbits = 18, L.sb = sorted_block_size(nnz, kSortSlab, value_bytes);
+    std::vector<int32_t> local, pk;
+    std::vector<int64_t> perm2;
+    if (L.sorted) {
+        local.resize(size_t(nnz)), pk.resize(size_t(nnz)), perm2.resize(size_t(nnz));
+        for (int64_t k = 0; k < nnz; ++k) local[size_t(k)] = L.idx[size_t(k)] - L.sstart[size_t(slab_of(L.idx[size_t(k)]))];
+    }
     for (int q = 0; q < kSlabs; ++q) {
         for (int i = 0; i <= rows; ++i)
             L.ptr[size_t(q) * (size_t(rows) + 1) + size_t(i)] = int32_t(soff[size_t(q)] + sp[size_t(q)][size_t(i)]);
-        std::vector<int32_t> b = row_blocks(rows, sp[size_t(q)].data());
+        const int32_t* qptr = L.ptr.data() + size_t(q) * (size_t(rows) + 1);
+        std::vector<int32_t> b = L.sorted ? sorted_blocks(rows, qptr, L.sb) : row_blocks(rows, sp[size_t(q)].data());
+        if (L.sorted) sort_blocks(b, qptr, local.data(), 0, L.perm.data(), L.cbits, pk, perm2);
         L.snblk[size_t(q)] = int(b.size()) - 1;
         L.sboff[size_t(q)] = int(L.blk.size());
         L.maxblk = std::max(L.maxblk, L.snblk[size_t(q)]);
         L.blk.insert(L.blk.end(), b.begin(), b.end());
     }
+    if (L.sorted) L.idx.swap(pk), L.perm.swap(perm2);
     return L;
 }
 
@@ -363,7 +555,7 @@ T* upload_values(Mem& mem, hipStream_t st, const HostLayout& L, const double* va
     T* d = mem.alloc<T>(size_t(std::max<int64_t>(nnz, 1)));
     if (nnz) {
         std::vector<T> h(static_cast<size_t>(nnz));
-        if (L.slab)
+        if (!L.perm.empty())
             for (int64_t k = 0; k < nnz; ++k) h[size_t(k)] = T(val[L.perm[size_t(k)]]);
         else
             for (int64_t k = 0; k < nnz; ++k) h[size_t(k)] = T(val[k]);
@@ -383,6 +575,8 @@ DevMat<T> upload_layout(Mem& mem, hipStream_t st, int rows, const HostLayout& L,
         return p;
     };
     d.ptr = up(L.ptr), d.idx = up(L.idx), d.blk = up(L.blk);
+    d.sorted = L.sorted, d.cbits = L.cbits, d.sb = L.sb;
+    if (L.sorted && L.slab) d.sstart = up(L.sstart);
     d.nch = int(L.chunk.size() / 3), d.nlr = int(L.lrow.size() / 3);
     if (d.nch) d.chunk = up(L.chunk), d.lrow = up(L.lrow), d.lpart = mem.alloc<T>(size_t(d.nch));
     if (L.slab) {
@@ -401,6 +595,7 @@ DevMat<U> share_structure(Mem& mem, hipStream_t st, const DevMat<T>& a, const Ho
     d.rows = a.rows, d.nblk = a.nblk, d.maxblk = a.maxblk, d.slab = a.slab;
     d.ptr = a.ptr, d.idx = a.idx, d.blk = a.blk, d.snblk = a.snblk, d.sboff = a.sboff;
     d.nch = a.nch, d.nlr = a.nlr, d.chunk = a.chunk, d.lrow = a.lrow;
+    d.sorted = a.sorted, d.cbits = a.cbits, d.sb = a.sb, d.sstart = a.sstart;
     if (a.nch) d.lpart = mem.alloc<U>(size_t(a.nch));
     d.val = upload_values<U>(mem, st, L, val, int64_t(L.idx.size()));
     if (a.slab) d.part = mem.alloc<U>(size_t(kSlabs) * size_t(std::max(a.rows, 1)));
@@ -410,12 +605,35 @@ DevMat<U> share_structure(Mem& mem, hipStream_t st, const DevMat<T>& a, const Ho
 template <class T>
 DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, int cols, const int32_t* ptr, const int32_t* idx,
                      const double* val) {
-    return upload_layout<T>(mem, st, rows, make_layout(rows, cols, ptr, idx), val);
+    return upload_layout<T>(mem, st, rows, make_layout(rows, cols, ptr, idx, int(sizeof(T))), val);
+}
+
+template <class T, int SB, class Epi>
+void launch_sorted(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
+    if (a.slab) {
+        if (a.maxblk > 0)
+            k_sorted_slab<T, SB><<<kSlabs * a.maxblk, kWG, 0, st>>>(a.rows, a.snblk, a.sboff, a.blk, a.ptr,
+                                                                   reinterpret_cast<const uint32_t*>(a.idx), a.val, v,
+                                                                   a.sstart, a.cbits, a.part);
+        k_combine_epi<T, Epi><<<grid_for(a.rows), kWG, 0, st>>>(a.rows, a.part, epi);
+        return;
+    }
+    if (a.nblk == 0) return;
+    k_sorted<T, SB, Epi><<<a.nblk + a.nch, kWG, 0, st>>>(a.nblk, a.blk, a.ptr, reinterpret_cast<const uint32_t*>(a.idx),
+                                                        a.val, v, a.cbits, epi, a.chunk, a.lpart);
+    if (a.nlr) k_long_rows<T, Epi><<<(a.nlr + kWG - 1) / kWG, kWG, 0, st>>>(a.nlr, a.lrow, a.lpart, epi);
 }
 
 template <class T, class Epi>
 void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
     if (a.rows == 0) return;
+    if (a.sorted) {
+        switch (a.sb) {
+            case 2048: launch_sorted<T, 2048>(a, v, epi, st); return;
+            case 4096: launch_sorted<T, 4096>(a, v, epi, st); return;
+            default: launch_sorted<T, 8192>(a, v, epi, st); return;
+        }
+    }
     if (a.slab) {
         if (a.maxblk > 0)
             k_slab_partial<T><<<kSlabs * a.maxblk, kWG, 0, st>>>(a.rows, a.snblk, a.sboff, a.blk, a.ptr, a.idx, a.val,

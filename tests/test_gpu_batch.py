@@ -2,10 +2,13 @@
 
 Each instance must take exactly the decisions a single fast-path solve takes. Row blocks, epilogues and the
 reduction partition are the single-instance ones (stage of 2048 nonzeros), so the batch is compared with
-single solves bit for bit: status, iterations, restarts, recentrings and the returned point. Beyond that: the
+single solves on the same (unsorted row-block) layout bit for bit: status, iterations, restarts, recentrings
+and the returned point. Beyond that: the
 planted optima, uneven sizes with inequality rows, a shared-matrix scenario family (stored once), warm starts
 with the cold-start fallback, and the autotuned stage size.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -13,6 +16,13 @@ from kkt_check import rel_kkt
 from taral_pdhg import ffi, gen
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def unsorted_single_layout(monkeypatch):
+    """The batch kernel uses the row-block layout of 2,048-nonzero stages; single solves compared with it bit for
+    bit are built on the same (unsorted) layout, which the TPDHG_LAYOUT switch selects."""
+    monkeypatch.setenv("TPDHG_LAYOUT", "unsorted")
 
 FAST = dict(tol=1e-6, precision_bits=32, halpern=1, refine=1)
 
