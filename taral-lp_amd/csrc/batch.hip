@@ -24,6 +24,10 @@
 namespace tpdhg {
 namespace {
 
+// Sorted pools pack (row-order position << kBatchColBits) | local column: instances need fewer than 2^19 columns
+// (and rows), and a block holds at most 2^13 nonzeros.
+constexpr int kBatchColBits = 19;
+
 template <class T>
 __device__ __forceinline__ T batch_lambda(int k) {
     return T(double(k + 1) / double(k + 2));
@@ -84,10 +88,12 @@ struct BEpiHCol {
     }
 };
 
-// Row-block SpMV over the batch (the k_rowblocks scheme of hipcommon.hpp with a CAP-entry stage). blk holds
-// (instance, first row, end row) triples; rows are local to the instance. With a list, workgroup b takes block
-// list[b] (nblk = list length), so a launch covers only the blocks of the instances still iterating.
-template <class T, int CAP, class Epi>
+// Row-block SpMV over the batch with a CAP-entry LDS stage. blk holds (instance, first row, end row) triples;
+// rows are local to the instance. SORTED: the sorted layout of hipcommon.hpp (k_sorted) per instance: within a
+// block the nonzeros are in column order with their row-order position packed above the column; otherwise the
+// k_rowblocks scheme. A row longer than CAP is a block of its own, summed strided by the workgroup. With a list,
+// workgroup b takes block list[b] (nblk = list length), so a launch covers only the instances still iterating.
+template <class T, int CAP, bool SORTED, class Epi>
 __global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict__ list, const int* __restrict__ blk,
                                                  const int* __restrict__ pbase, const int* __restrict__ ptr,
                                                  const int* __restrict__ idx, const T* __restrict__ val,
@@ -103,10 +109,11 @@ __global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict
     const T* vi = v + voff[inst];
     const int tid = int(threadIdx.x);
     const int k0 = P[r0], k1 = P[r1];
+    constexpr uint32_t kMask = SORTED ? (1u << kBatchColBits) - 1u : 0xffffffffu;
     epi.load(inst);
     if (k1 - k0 > CAP) {
         T s = 0;
-        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * vi[idx[k]];
+        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * vi[uint32_t(idx[k]) & kMask];
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
@@ -116,9 +123,25 @@ __global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict
 #pragma unroll
     for (int u = 0; u < CAP / kWG; ++u) {
         const int k = k0 + u * kWG + tid;
-        if (k < k1) sh[k - k0] = val[k] * vi[idx[k]];
+        if (k < k1) {
+            if constexpr (SORTED) {
+                const uint32_t p = uint32_t(idx[k]);
+                sh[p >> kBatchColBits] = val[k] * vi[p & kMask];
+            } else {
+                sh[k - k0] = val[k] * vi[idx[k]];
+            }
+        }
     }
     __syncthreads();
+    if constexpr (SORTED) {  // as hipcommon.hpp's k_sorted: one thread per row, CSR order
+        for (int r = r0 + tid; r < r1; r += kWG) {
+            const int a = P[r] - k0, e = P[r + 1] - k0;
+            T s = 0;
+            for (int j = a; j < e; ++j) s += sh[j];
+            epi(r, s);
+        }
+        return;
+    }
     const int nr = r1 - r0;
     int lg = 6;
     while (lg > 0 && (nr << lg) > kWG) --lg;
@@ -593,6 +616,9 @@ private:
             mat_of_[size_t(i)] = found;
         }
         if (mt > INT32_MAX || nt > INT32_MAX) throw std::runtime_error("batch too large for 32-bit vector offsets");
+        sorted_ = sorted_layout_enabled();
+        for (int i = 0; i < B_; ++i)
+            if (m_[size_t(i)] >= (1 << kBatchColBits) || n_[size_t(i)] >= (1 << kBatchColBits)) sorted_ = false;
         mtot_ = int(mt), ntot_ = int(nt);
         auto add = [](Pool& pool, const Csr& a) {
             const int64_t at = int64_t(pool.val.size());
@@ -708,7 +734,32 @@ private:
     }
 
     // Block tables of both sides for stage size `cap` (device copies; the previous ones stay allocated).
+    // Sorted pools for stage `cap`: every block's nonzeros in column order with packed positions, values permuted
+    // the same way; uploaded over the side's index and value arrays (and the FP64 twin's values).
+    void repack(const Pool& pool, const std::vector<std::vector<int32_t>>& per, bool a_side) {
+        const size_t nnz = pool.val.size();
+        std::vector<int32_t> pk(nnz);
+        std::vector<int64_t> perm(nnz);
+        for (size_t u = 0; u < pool.rows.size(); ++u) {
+            // blocks hold local rows; pool pointers are absolute, so sort_blocks sees absolute nonzero ranges
+            sort_blocks(per[u], pool.ptr.data() + pool.pbase[u], pool.idx.data(), 0, nullptr, kBatchColBits, pk, perm,
+                        cap_candidate_);
+        }
+        std::vector<double> pv(nnz);
+        for (size_t k = 0; k < nnz; ++k) pv[k] = pool.val[size_t(perm[k])];
+        Side& sd = a_side ? A_ : AT_;
+        Side64& s64 = a_side ? A64_ : AT64_;
+        if (nnz) HIP_OK(hipMemcpyAsync(sd.idx, pk.data(), nnz * sizeof(int), hipMemcpyHostToDevice, st_));
+        std::vector<T> tv(nnz);
+        for (size_t k = 0; k < nnz; ++k) tv[k] = T(pv[k]);
+        if (nnz) HIP_OK(hipMemcpyAsync(sd.val, tv.data(), nnz * sizeof(T), hipMemcpyHostToDevice, st_));
+        if constexpr (sizeof(T) != sizeof(double))
+            if (nnz) HIP_OK(hipMemcpyAsync(s64.val, pv.data(), nnz * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+    }
+
     void set_cap(int cap) {
+        cap_candidate_ = cap;
         auto table = [&](const Pool& pool) {
             std::vector<std::vector<int32_t>> per(pool.rows.size());
             for (size_t u = 0; u < pool.rows.size(); ++u) {
@@ -716,8 +767,10 @@ private:
                 std::vector<int32_t> lp(static_cast<size_t>(pool.rows[u]) + 1);
                 const int32_t b0 = pool.ptr[size_t(pool.pbase[u])];
                 for (int r = 0; r <= pool.rows[u]; ++r) lp[size_t(r)] = pool.ptr[size_t(pool.pbase[u]) + size_t(r)] - b0;
-                per[u] = row_blocks_cap(pool.rows[u], lp.data(), cap);
+                per[u] = sorted_ ? sorted_blocks(pool.rows[u], lp.data(), cap, cap)
+                                 : row_blocks_cap(pool.rows[u], lp.data(), cap);
             }
+            if (sorted_) repack(pool, per, pool.val.data() == PA_.val.data());
             std::vector<int32_t> t;
             for (int i = 0; i < B_; ++i) {
                 const std::vector<int32_t>& b = per[size_t(mat_of_[size_t(i)])];
@@ -811,19 +864,23 @@ private:
                    int nlist = 0) {
         const int nb = list ? nlist : side.nblk;
         if (nb == 0) return;
-        switch (cap_) {
-            case 1024:
-                k_bblocks<U, 1024, Epi><<<nb, kWG, 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val,
-                                                            side.voff, v, on, epi);
-                break;
-            case 4096:
-                k_bblocks<U, 4096, Epi><<<nb, kWG, 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val,
-                                                            side.voff, v, on, epi);
-                break;
-            default:
-                k_bblocks<U, 2048, Epi><<<nb, kWG, 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val,
-                                                            side.voff, v, on, epi);
+#define TPDHG_BB(CAPV, SRT)                                                                                      \
+    k_bblocks<U, CAPV, SRT, Epi><<<nb, kWG, 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val, \
+                                                      side.voff, v, on, epi)
+        if (sorted_) {
+            switch (cap_) {
+                case 1024: TPDHG_BB(1024, true); break;
+                case 4096: TPDHG_BB(4096, true); break;
+                default: TPDHG_BB(2048, true);
+            }
+        } else {
+            switch (cap_) {
+                case 1024: TPDHG_BB(1024, false); break;
+                case 4096: TPDHG_BB(4096, false); break;
+                default: TPDHG_BB(2048, false);
+            }
         }
+#undef TPDHG_BB
     }
     // Live-block lists of both sides for the instances in `on`: rebuilt when an instance in `on` is missing from
     // the current lists or when the live blocks have dropped to half the lists' length (so at most ~log2 of the
@@ -914,7 +971,8 @@ private:
     int64_t nnz_total_ = 0;
     Pool PA_, PT_;
     std::vector<int32_t> rtab_, ctab_, rfirst_, rcount_, cfirst_, ccount_;
-    int nrseg_ = 0, ncseg_ = 0, cap_ = kCap;
+    int nrseg_ = 0, ncseg_ = 0, cap_ = kCap, cap_candidate_ = kCap;
+    bool sorted_ = false;  // sorted pools (every instance below 2^19 rows and columns)
     std::map<int, std::pair<int*, int*>> tables_;
     std::vector<int32_t> afirst_, acount_, tfirst_, tcount_;
     std::vector<uint8_t> list_inst_;

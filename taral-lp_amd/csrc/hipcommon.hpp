@@ -356,18 +356,19 @@ std::vector<int32_t> row_blocks(int rows, const int32_t* ptr) {
     return b;
 }
 
-// Row blocks of the sorted layout: maximal runs of rows with at most sb nonzeros, no row longer than kCap inside
-// a run; a row longer than kCap forms a block of its own.
-std::vector<int32_t> sorted_blocks(int rows, const int32_t* ptr, int sb) {
+// Row blocks of the sorted layout: maximal runs of rows with at most sb nonzeros, no row longer than long_above
+// inside a run; a row longer than long_above forms a block of its own (long_above <= sb).
+std::vector<int32_t> sorted_blocks(int rows, const int32_t* ptr, int sb, int long_above = kCap) {
     std::vector<int32_t> b;
     b.push_back(0);
     int r0 = 0;
     while (r0 < rows) {
         int r = r0;
-        if (int64_t(ptr[r0 + 1]) - ptr[r0] > kCap) {
+        if (int64_t(ptr[r0 + 1]) - ptr[r0] > long_above) {
             r = r0 + 1;
         } else {
-            while (r < rows && int64_t(ptr[r + 1]) - ptr[r] <= kCap && int64_t(ptr[r + 1]) - ptr[r0] <= sb) ++r;
+            while (r < rows && int64_t(ptr[r + 1]) - ptr[r] <= long_above && int64_t(ptr[r + 1]) - ptr[r0] <= sb) ++r;
+            if (r == r0) r = r0 + 1;
         }
         b.push_back(r);
         r0 = r;
@@ -454,14 +455,15 @@ bool sorted_layout_enabled() {
 // Sorts the entries of every block of a row range by column, writing packed words and the permutation (entries
 // [base + k0, base + k1) of idx_in map to the same range of pk / perm; positions are relative to the block).
 void sort_blocks(const std::vector<int32_t>& blk, const int32_t* ptr, const int32_t* cols_local, int64_t perm_base_in,
-                 const int64_t* perm_in, int cbits, std::vector<int32_t>& pk, std::vector<int64_t>& perm) {
+                 const int64_t* perm_in, int cbits, std::vector<int32_t>& pk, std::vector<int64_t>& perm,
+                 int long_above = kCap) {
     std::vector<int32_t> order;
     for (size_t b = 0; b + 1 < blk.size(); ++b) {
         const int32_t k0 = ptr[blk[b]], k1 = ptr[blk[b + 1]];
         const int32_t len = k1 - k0;
         order.resize(size_t(len));
         for (int32_t i = 0; i < len; ++i) order[size_t(i)] = i;
-        const bool single_long = blk[b + 1] - blk[b] == 1 && len > kCap;
+        const bool single_long = blk[b + 1] - blk[b] == 1 && len > long_above;
         if (!single_long)
             std::stable_sort(order.begin(), order.end(),
                              [&](int32_t a, int32_t c) { return cols_local[k0 + a] < cols_local[k0 + c]; });

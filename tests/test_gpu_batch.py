@@ -2,8 +2,8 @@
 
 Each instance must take exactly the decisions a single fast-path solve takes. Row blocks, epilogues and the
 reduction partition are the single-instance ones (stage of 2048 nonzeros), so the batch is compared with
-single solves on the same (unsorted row-block) layout bit for bit: status, iterations, restarts, recentrings
-and the returned point. Beyond that: the
+single solves on the same layout (column-sorted blocks by default, and the unsorted row blocks) bit for bit:
+status, iterations, restarts, recentrings and the returned point. Beyond that: the
 planted optima, uneven sizes with inequality rows, a shared-matrix scenario family (stored once), warm starts
 with the cold-start fallback, and the autotuned stage size.
 """
@@ -18,11 +18,11 @@ from taral_pdhg import ffi, gen
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True)
-def unsorted_single_layout(monkeypatch):
-    """The batch kernel uses the row-block layout of 2,048-nonzero stages; single solves compared with it bit for
-    bit are built on the same (unsorted) layout, which the TPDHG_LAYOUT switch selects."""
-    monkeypatch.setenv("TPDHG_LAYOUT", "unsorted")
+@pytest.fixture(params=["sorted", "unsorted"], autouse=True)
+def layout(request, monkeypatch):
+    """Both row-block layouts (TPDHG_LAYOUT): the batch and the single solves it is compared with use the same one."""
+    monkeypatch.setenv("TPDHG_LAYOUT", request.param)
+    return request.param
 
 FAST = dict(tol=1e-6, precision_bits=32, halpern=1, refine=1)
 
