@@ -133,13 +133,8 @@ __global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict
         }
     }
     __syncthreads();
-    if constexpr (SORTED) {  // as hipcommon.hpp's k_sorted: one thread per row, CSR order
-        for (int r = r0 + tid; r < r1; r += kWG) {
-            const int a = P[r] - k0, e = P[r + 1] - k0;
-            T s = 0;
-            for (int j = a; j < e; ++j) s += sh[j];
-            epi(r, s);
-        }
+    if constexpr (SORTED) {  // as hipcommon.hpp's k_sorted
+        block_row_sums<T>(P, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });
         return;
     }
     const int nr = r1 - r0;
