@@ -18,6 +18,13 @@ from oracle import oracle
 from taral_pdhg import ffi, gen
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=["sorted", "unsorted"], autouse=True)
+def layout(request, monkeypatch):
+    """Every fast-path check on both SpMV layouts (the small test problems default to row blocks)."""
+    monkeypatch.setenv("TPDHG_LAYOUT", request.param)
+    return request.param
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 

@@ -1,7 +1,8 @@
 """HIP kernels against the CPU oracle, through the C ABI (libtaral_pdhg.so).
 
-SpMV: FP64 within summation-order error of the FP64 product; FP32 within (k + 2) 2^-24 sum |a_k x_k| of the
-product of the FP32-rounded inputs (k = row length). Backend operations (power iteration, start, one
+SpMV (both layouts: column-sorted blocks and row blocks, plain and XCD-slab, long rows): FP64 within
+summation-order error of the FP64 product; FP32 within (k + 2) 2^-24 sum |a_k x_k| of the product of the
+FP32-rounded inputs (k = row length). Backend operations (power iteration, start, one
 64-step iteration block, fused current+average evaluation, restart, download) against the oracle's CPU
 backend on the same scaled problem: FP64 to 1e-10 relative (summation order is the only difference).
 """
@@ -45,15 +46,17 @@ SPMV_CASES = {  # row lengths; gathered vectors of 2^19 entries or more take the
 }
 
 
+@pytest.mark.parametrize("layout", ["sorted", "unsorted"])
 @pytest.mark.parametrize("bits", [64, 32])
 @pytest.mark.parametrize("case", sorted(SPMV_CASES))
-def test_spmv_parity(case, bits):
+def test_spmv_parity(case, bits, layout, monkeypatch):
+    monkeypatch.setenv("TPDHG_LAYOUT", layout)  # both layouts: column-sorted blocks and row blocks
     rng = np.random.default_rng(11)
     lengths = SPMV_CASES[case](rng)
     cols = 600_000 if case.startswith("slab") else 12000
     ptr, idx, val = _csr_from_rows(lengths, cols, rng)
     v = rng.standard_normal(cols)
-    print(f"spmv case={case} bits={bits} rows={ptr.size - 1} nnz={ptr[-1]}", flush=True)
+    print(f"spmv case={case} bits={bits} layout={layout} rows={ptr.size - 1} nnz={ptr[-1]}", flush=True)
     y, _, _ = ffi.spmv_bench(ptr, idx, val, v, precision_bits=bits, reps=2)
     if bits == 64:
         ref, mag = _spmv_ref(ptr, idx, val, v)
