@@ -356,7 +356,7 @@ private:
             d.val = reinterpret_cast<double*>(a.val), d.part = reinterpret_cast<double*>(a.part);
             d.nch = a.nch, d.nlr = a.nlr, d.chunk = a.chunk, d.lrow = a.lrow;
             d.lpart = reinterpret_cast<double*>(a.lpart);
-            d.sorted = a.sorted, d.cbits = a.cbits, d.sb = a.sb, d.sstart = a.sstart;
+            d.sorted = a.sorted, d.cbits = a.cbits, d.sb = a.sb, d.sstart = a.sstart, d.nslab = a.nslab;
             return d;
         } else {
             return share_structure<double>(mem_, st_, a, L, h.val.data());

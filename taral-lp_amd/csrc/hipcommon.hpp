@@ -180,14 +180,14 @@ __global__ __launch_bounds__(kWG) void k_sorted(int nblk, const int* __restrict_
 // Sorted slab layout: workgroup b takes row block b / 8 of slab b % 8 (columns local to the slab, gathered from
 // v + sstart[slab]) and writes the slab's partial row sums part[slab * rows + row]; same staging as k_sorted.
 template <class T, int SB>
-__global__ __launch_bounds__(kWG) void k_sorted_slab(int rows, const int* __restrict__ snblk,
+__global__ __launch_bounds__(kWG) void k_sorted_slab(int rows, int nslab, const int* __restrict__ snblk,
                                                      const int* __restrict__ sboff, const int* __restrict__ blk_all,
                                                      const int* __restrict__ ptr_all, const uint32_t* __restrict__ pk,
                                                      const T* __restrict__ val, const T* __restrict__ v,
                                                      const int* __restrict__ sstart, int cbits, T* __restrict__ part) {
     __shared__ T sh[SB];
     __shared__ T red[kWG / 64];
-    const int q = int(blockIdx.x) & (kSlabsK - 1), lb = int(blockIdx.x) / kSlabsK;
+    const int q = int(blockIdx.x) % nslab, lb = int(blockIdx.x) / nslab;
     if (lb >= snblk[q]) return;
     const int* blk = blk_all + sboff[q];
     const int* ptr = ptr_all + size_t(q) * (size_t(rows) + 1);
@@ -223,14 +223,14 @@ __global__ __launch_bounds__(kWG) void k_sorted_slab(int rows, const int* __rest
 // Slab layout, first kernel: workgroup b handles row block b / 8 of slab b % 8 (slab = XCD) and writes the
 // slab's partial row sums part[slab * rows + row]; same staging and reduction as k_rowblocks.
 template <class T>
-__global__ __launch_bounds__(kWG) void k_slab_partial(int rows, const int* __restrict__ snblk,
+__global__ __launch_bounds__(kWG) void k_slab_partial(int rows, int nslab, const int* __restrict__ snblk,
                                                       const int* __restrict__ sboff, const int* __restrict__ blk_all,
                                                       const int* __restrict__ ptr_all, const int* __restrict__ idx,
                                                       const T* __restrict__ val, const T* __restrict__ v,
                                                       T* __restrict__ part) {
     __shared__ T sh[kCap];
     __shared__ T red[kWG / 64];
-    const int q = int(blockIdx.x) & (kSlabsK - 1), lb = int(blockIdx.x) / kSlabsK;
+    const int q = int(blockIdx.x) % nslab, lb = int(blockIdx.x) / nslab;
     if (lb >= snblk[q]) return;
     const int* blk = blk_all + sboff[q];
     const int* ptr = ptr_all + size_t(q) * (size_t(rows) + 1);
@@ -268,12 +268,11 @@ __global__ __launch_bounds__(kWG) void k_slab_partial(int rows, const int* __res
 
 // Slab layout, second kernel: the eight partial sums of each row in slab order, then the epilogue.
 template <class T, class Epi>
-__global__ __launch_bounds__(kWG) void k_combine_epi(int rows, const T* __restrict__ part, Epi epi) {
+__global__ __launch_bounds__(kWG) void k_combine_epi(int rows, int nslab, const T* __restrict__ part, Epi epi) {
     epi.load();
     for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < rows; i += int(gridDim.x * blockDim.x)) {
         T s = part[i];
-#pragma unroll
-        for (int q = 1; q < kSlabsK; ++q) s += part[size_t(q) * size_t(rows) + size_t(i)];
+        for (int q = 1; q < nslab; ++q) s += part[size_t(q) * size_t(rows) + size_t(i)];
         epi(i, s);
     }
 }
@@ -434,6 +433,7 @@ struct DevMat {
     bool sorted = false;                                 // idx holds packed (position, column) words
     int cbits = 0, sb = 0;
     int* sstart = nullptr;                               // sorted slab: first column of each slab
+    int nslab = 0;                                       // slab: number of column slabs (a divisor of 8)
 };
 
 struct HostLayout {
@@ -445,10 +445,25 @@ struct HostLayout {
     bool sorted = false;
     int cbits = 0, sb = 0;  // sorted: column bits of the packed words, nonzeros per block (power of two)
     std::vector<int32_t> sstart;
+    int nslab = 0;
 };
 
+// Column slabs of the slab layout: the fewest of 2, 4, 8 that keep one slab's slice of the gathered vector within
+// 1 MiB of FP32 (measured at config 1: A x gathering 2M entries is fastest with 8 slabs, 207 us against 225 with 4;
+// A^T z gathering 1M entries with 4, 205 us against 218 with 8). Workgroup b works on slab b mod nslab, so with
+// fewer than 8 slabs XCDs x and x + nslab share one. TPDHG_SLABS=2|4|8 forces a count.
+int slab_count(int cols) {
+    if (const char* e = std::getenv("TPDHG_SLABS")) {
+        const int k = std::atoi(e);
+        if (k == 2 || k == 4 || k == 8) return k;
+    }
+    int k = 2;
+    while (k < kSlabs && int64_t(cols) * 4 > int64_t(k) << 20) k *= 2;
+    return k;
+}
+
 // Largest sorted blocks (32 KB of FP32 products in LDS; measured at config 1: 8,192 beats 4,096 and 2,048 and
-// 16,384 loses to occupancy). Plain layout: 13 position bits for columns < 2^19; slab: 14 bits, slab columns < 2^18.
+// 16,384 loses to occupancy). Plain layout: 13 position bits for columns < 2^19; slabs: 13 or 14 bits.
 constexpr int kSortPlain = 8192;
 constexpr int kSortSlab = 8192;
 constexpr int kSortMinBlocks = 1024;  // the block size shrinks (down to kCap) until there are this many blocks
@@ -530,18 +545,20 @@ HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* id
         }
         return L;
     }
-    auto slab_of = [&](int32_t c) { return int(int64_t(c) * kSlabs / cols); };
-    std::vector<std::vector<int32_t>> sp(kSlabs, std::vector<int32_t>(size_t(rows) + 1, 0));
+    const int NS = slab_count(cols);
+    L.nslab = NS;
+    auto slab_of = [&](int32_t c) { return int(int64_t(c) * NS / cols); };
+    std::vector<std::vector<int32_t>> sp(NS, std::vector<int32_t>(size_t(rows) + 1, 0));
     for (int i = 0; i < rows; ++i)
         for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) sp[size_t(slab_of(idx[k]))][size_t(i) + 1]++;
-    std::vector<int64_t> soff(kSlabs + 1, 0);
-    for (int q = 0; q < kSlabs; ++q) {
+    std::vector<int64_t> soff(NS + 1, 0);
+    for (int q = 0; q < NS; ++q) {
         for (int i = 0; i < rows; ++i) sp[size_t(q)][size_t(i) + 1] += sp[size_t(q)][size_t(i)];
         soff[size_t(q) + 1] = soff[size_t(q)] + sp[size_t(q)][size_t(rows)];
     }
     L.idx.resize(size_t(nnz)), L.perm.resize(size_t(nnz));
-    std::vector<int64_t> next(kSlabs * size_t(rows));
-    for (int q = 0; q < kSlabs; ++q)
+    std::vector<int64_t> next(NS * size_t(rows));
+    for (int q = 0; q < NS; ++q)
         for (int i = 0; i < rows; ++i) next[size_t(q) * size_t(rows) + size_t(i)] = soff[size_t(q)] + sp[size_t(q)][size_t(i)];
     for (int i = 0; i < rows; ++i)
         for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
@@ -549,22 +566,22 @@ HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* id
             const int64_t at = next[size_t(q) * size_t(rows) + size_t(i)]++;
             L.idx[size_t(at)] = idx[k], L.perm[size_t(at)] = k;
         }
-    L.ptr.resize(size_t(kSlabs) * (size_t(rows) + 1));
-    L.snblk.resize(kSlabs), L.sboff.resize(kSlabs);
-    // slab q holds the columns c with c * kSlabs / cols == q, i.e. [ceil(q * cols / kSlabs), ...)
-    L.sstart.resize(kSlabs + 1);
-    for (int q = 0; q <= kSlabs; ++q) L.sstart[size_t(q)] = int32_t((int64_t(q) * cols + kSlabs - 1) / kSlabs);
+    L.ptr.resize(size_t(NS) * (size_t(rows) + 1));
+    L.snblk.resize(NS), L.sboff.resize(NS);
+    // slab q holds the columns c with c * NS / cols == q, i.e. [ceil(q * cols / NS), ...)
+    L.sstart.resize(NS + 1);
+    for (int q = 0; q <= NS; ++q) L.sstart[size_t(q)] = int32_t((int64_t(q) * cols + NS - 1) / NS);
     int64_t width = 0;
-    for (int q = 0; q < kSlabs; ++q) width = std::max<int64_t>(width, L.sstart[size_t(q) + 1] - L.sstart[size_t(q)]);
-    L.sorted = want_sorted && width <= (1 << 18);
-    if (L.sorted) L.cbits = 18, L.sb = sorted_block_size(nnz, kSortSlab, value_bytes);
+    for (int q = 0; q < NS; ++q) width = std::max<int64_t>(width, L.sstart[size_t(q) + 1] - L.sstart[size_t(q)]);
+    L.sorted = want_sorted && width <= (1 << 19);
+    if (L.sorted) L.cbits = width <= (1 << 18) ? 18 : 19, L.sb = sorted_block_size(nnz, kSortSlab, value_bytes);
     std::vector<int32_t> local, pk;
     std::vector<int64_t> perm2;
     if (L.sorted) {
         local.resize(size_t(nnz)), pk.resize(size_t(nnz)), perm2.resize(size_t(nnz));
         for (int64_t k = 0; k < nnz; ++k) local[size_t(k)] = L.idx[size_t(k)] - L.sstart[size_t(slab_of(L.idx[size_t(k)]))];
     }
-    for (int q = 0; q < kSlabs; ++q) {
+    for (int q = 0; q < NS; ++q) {
         for (int i = 0; i <= rows; ++i)
             L.ptr[size_t(q) * (size_t(rows) + 1) + size_t(i)] = int32_t(soff[size_t(q)] + sp[size_t(q)][size_t(i)]);
         const int32_t* qptr = L.ptr.data() + size_t(q) * (size_t(rows) + 1);
@@ -604,13 +621,13 @@ DevMat<T> upload_layout(Mem& mem, hipStream_t st, int rows, const HostLayout& L,
         return p;
     };
     d.ptr = up(L.ptr), d.idx = up(L.idx), d.blk = up(L.blk);
-    d.sorted = L.sorted, d.cbits = L.cbits, d.sb = L.sb;
+    d.sorted = L.sorted, d.cbits = L.cbits, d.sb = L.sb, d.nslab = L.nslab;
     if (L.sorted && L.slab) d.sstart = up(L.sstart);
     d.nch = int(L.chunk.size() / 3), d.nlr = int(L.lrow.size() / 3);
     if (d.nch) d.chunk = up(L.chunk), d.lrow = up(L.lrow), d.lpart = mem.alloc<T>(size_t(d.nch));
     if (L.slab) {
         d.snblk = up(L.snblk), d.sboff = up(L.sboff);
-        d.part = mem.alloc<T>(size_t(kSlabs) * size_t(std::max(rows, 1)));
+        d.part = mem.alloc<T>(size_t(L.nslab) * size_t(std::max(rows, 1)));
     }
     d.val = upload_values<T>(mem, st, L, val, int64_t(L.idx.size()));
     HIP_OK(hipStreamSynchronize(st));
@@ -624,10 +641,10 @@ DevMat<U> share_structure(Mem& mem, hipStream_t st, const DevMat<T>& a, const Ho
     d.rows = a.rows, d.nblk = a.nblk, d.maxblk = a.maxblk, d.slab = a.slab;
     d.ptr = a.ptr, d.idx = a.idx, d.blk = a.blk, d.snblk = a.snblk, d.sboff = a.sboff;
     d.nch = a.nch, d.nlr = a.nlr, d.chunk = a.chunk, d.lrow = a.lrow;
-    d.sorted = a.sorted, d.cbits = a.cbits, d.sb = a.sb, d.sstart = a.sstart;
+    d.sorted = a.sorted, d.cbits = a.cbits, d.sb = a.sb, d.sstart = a.sstart, d.nslab = a.nslab;
     if (a.nch) d.lpart = mem.alloc<U>(size_t(a.nch));
     d.val = upload_values<U>(mem, st, L, val, int64_t(L.idx.size()));
-    if (a.slab) d.part = mem.alloc<U>(size_t(kSlabs) * size_t(std::max(a.rows, 1)));
+    if (a.slab) d.part = mem.alloc<U>(size_t(a.nslab) * size_t(std::max(a.rows, 1)));
     return d;
 }
 
@@ -641,10 +658,10 @@ template <class T, int SB, class Epi>
 void launch_sorted(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
     if (a.slab) {
         if (a.maxblk > 0)
-            k_sorted_slab<T, SB><<<kSlabs * a.maxblk, kWG, 0, st>>>(a.rows, a.snblk, a.sboff, a.blk, a.ptr,
+            k_sorted_slab<T, SB><<<a.nslab * a.maxblk, kWG, 0, st>>>(a.rows, a.nslab, a.snblk, a.sboff, a.blk, a.ptr,
                                                                    reinterpret_cast<const uint32_t*>(a.idx), a.val, v,
                                                                    a.sstart, a.cbits, a.part);
-        k_combine_epi<T, Epi><<<grid_for(a.rows), kWG, 0, st>>>(a.rows, a.part, epi);
+        k_combine_epi<T, Epi><<<grid_for(a.rows), kWG, 0, st>>>(a.rows, a.nslab, a.part, epi);
         return;
     }
     if (a.nblk == 0) return;
@@ -665,9 +682,9 @@ void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st)
     }
     if (a.slab) {
         if (a.maxblk > 0)
-            k_slab_partial<T><<<kSlabs * a.maxblk, kWG, 0, st>>>(a.rows, a.snblk, a.sboff, a.blk, a.ptr, a.idx, a.val,
-                                                                v, a.part);
-        k_combine_epi<T, Epi><<<grid_for(a.rows), kWG, 0, st>>>(a.rows, a.part, epi);
+            k_slab_partial<T><<<a.nslab * a.maxblk, kWG, 0, st>>>(a.rows, a.nslab, a.snblk, a.sboff, a.blk, a.ptr,
+                                                                 a.idx, a.val, v, a.part);
+        k_combine_epi<T, Epi><<<grid_for(a.rows), kWG, 0, st>>>(a.rows, a.nslab, a.part, epi);
         return;
     }
     if (a.nblk == 0) return;
