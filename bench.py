@@ -153,6 +153,8 @@ def solver_opts(a, mode_name, **extra):
              verbose=int(a.verbose), progress_interval=a.progress, **mode_options(mode_name))
     if a.omega_rule is not None:
         o["omega_rule"] = a.omega_rule
+    if a.reflection is not None and o.get("halpern"):
+        o["reflection"] = a.reflection
     o.update(extra)
     return o
 
@@ -414,6 +416,8 @@ def main():
     ap.add_argument("--fp64-sample", type=int, default=16, help="batch workloads: LPs solved one by one in FP64")
     ap.add_argument("--batch-count", type=int, default=512, help="batch workloads: number of LPs")
     ap.add_argument("--cap", type=int, default=0, help="batch row-block stage (0 = autotune)")
+    ap.add_argument("--reflection", type=float, default=None,
+                    help="fast path: Halpern reflection coefficient in [1, 2) (default: the library's)")
     ap.add_argument("--omega-rule", type=int, default=None,
                     help="fast path: primal-weight guard (0 reference update, 1 bounded, 2 balance, 3 both; "
                          "default: the library's, 3)")
