@@ -164,7 +164,7 @@ typedef struct {
     int32_t profile_kernels; /* HIP-event timing of the row/column kernels */
     int32_t halpern;   /* 1: fast path, reflected Halpern PDHG on an FP64-recentred problem (no reference
                           counterpart; the reference algorithm is the default, halpern = 0) */
-    double reflection; /* Halpern reflection coefficient in [1, 2) (default 1.95) */
+    double reflection; /* Halpern reflection coefficient in [1, 2) (default 1.9) */
     double primal_weight; /* initial primal weight of a warm-started solve, 0 = ||c||/||b|| as for a cold start
                              (no reference counterpart: the reference always cold-starts, gpu/pdhg.cu:450-458) */
     int32_t warm_fallback; /* 1: keep a warm start only when its relative KKT is below the cold start's */

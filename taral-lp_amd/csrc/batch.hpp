@@ -58,7 +58,7 @@ struct BatchOptions {
     int precision_bits = 32;
     int eval_freq = 64;
     bool use_graph = true;
-    double rho = 1.95;
+    double rho = 1.9;
     int cap = 0;  // nonzeros staged per row block: 1024, 2048 or 4096; 0 = pick by timing (autotune)
 };
 

@@ -182,7 +182,7 @@ int main(int argc, char** argv) {
         std::fprintf(stderr,
                      "usage: pdhg MODEL.mps --device gpu|cpu [--threads N] [--tol 1e-4] [--time-limit S] "
                      "[--max-iter N] [--json OUT.json] [--sol OUT.sol] [--dual OUT.dual] [--fp32] [--eval-freq 64] "
-                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.95] [--refine] "
+                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--refine] "
                      "[--max-refinements 8] "
                      "[--profile-kernels] [--warm-sol X.sol] [--warm-dual X.dual] [--recheck]\n"
                      "       pdhg --batch LIST.txt [--json OUT.jsonl] [solver flags]\n");

@@ -108,7 +108,7 @@ struct FastOptions {
     int precision_bits = 32;  // iterate type: 32 (FP32 iterates and SpMV, FP64 evaluation and base) or 64
     int eval_freq = 64;
     bool use_graph = true;
-    double rho = 1.95;  // reflection coefficient of the Halpern iteration, in [1, 2)
+    double rho = 1.9;  // reflection coefficient of the Halpern iteration, in [1, 2)
 };
 
 std::unique_ptr<FastDevice> make_fast_device(const Scaled& P, const FastOptions& opt);

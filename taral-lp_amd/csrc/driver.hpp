@@ -44,7 +44,7 @@ struct Options {  // defaults of gpu/pdhg.cu:744-750 plus this build's additions
     int max_refinements = 1000000000;  // cap on the recentrings
     bool profile_kernels = false;
     bool halpern = false;      // fast path: reflected Halpern PDHG on the FP64-recentred problem
-    double rho = 1.95;         // its reflection coefficient
+    double rho = 1.9;          // its reflection coefficient
     double omega0 = 0;         // initial primal weight of a warm-started solve (0: ||c||/||b|| as for a cold start)
     bool warm_fallback = false;  // cold-start instead when the warm point's relative KKT is not below the cold one's
     double progress_s = 0;       // > 0: a progress line on stderr every progress_s seconds of solving
