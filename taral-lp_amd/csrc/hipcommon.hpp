@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kWG) void k_sumsq(int n, const T* __restrict__ x, d
 
 // out[k] = sum over b < blocks of part[(b0 + b) * K + k] for k < K: each thread sums a fixed strided subset of
 // the partials, then a fixed-order tree in LDS, so the result is the same on every run.
-__global__ __launch_bounds__(kWG) void k_finish(int b0, int blocks, int K, const double* __restrict__ part,
+[[maybe_unused]] __global__ __launch_bounds__(kWG) void k_finish(int b0, int blocks, int K, const double* __restrict__ part,
                                                 double* __restrict__ out) {
     __shared__ double sh[kWG];
     const int t = int(threadIdx.x);
@@ -392,8 +392,8 @@ std::vector<int32_t> sorted_blocks(int rows, const int32_t* ptr, int sb, int lon
     return b;
 }
 
-int grid_for(int k) { return std::max(1, std::min(2048, (k + kWG - 1) / kWG)); }
-int red_blocks(int k) { return k > 0 ? std::max(1, std::min(kMaxRedBlocks, (k + kWG - 1) / kWG)) : 0; }
+[[maybe_unused]] int grid_for(int k) { return std::max(1, std::min(2048, (k + kWG - 1) / kWG)); }
+[[maybe_unused]] int red_blocks(int k) { return k > 0 ? std::max(1, std::min(kMaxRedBlocks, (k + kWG - 1) / kWG)) : 0; }
 
 struct Mem {  // device allocations owned by one backend
     std::vector<void*> ptrs;
@@ -491,7 +491,7 @@ LayoutMode layout_mode() {
     return LayoutMode::automatic;
 }
 
-bool sorted_layout_enabled() { return layout_mode() != LayoutMode::unsorted; }
+[[maybe_unused]] bool sorted_layout_enabled() { return layout_mode() != LayoutMode::unsorted; }
 
 // Sorts the entries of every block of a row range by column, writing packed words and the permutation (entries
 // [base + k0, base + k1) of idx_in map to the same range of pk / perm; positions are relative to the block).
@@ -517,7 +517,8 @@ void sort_blocks(const std::vector<int32_t>& blk, const int32_t* ptr, const int3
     }
 }
 
-HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* idx, int value_bytes = 4) {
+[[maybe_unused]] HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* idx,
+                                         int value_bytes = 4) {
     HostLayout L;
     const int64_t nnz = ptr[rows];
     L.slab = cols >= kSlabMinCols && rows > 0;
@@ -692,7 +693,7 @@ void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st)
     if (a.nlr) k_long_rows<T, Epi><<<(a.nlr + kWG - 1) / kWG, kWG, 0, st>>>(a.nlr, a.lrow, a.lpart, epi);
 }
 
-std::string device_name() {
+[[maybe_unused]] std::string device_name() {
     int dev = 0;
     HIP_OK(hipGetDevice(&dev));
     hipDeviceProp_t prop;

@@ -27,7 +27,6 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {  // round to nearest even
     u += 0x7fffu + ((u >> 16) & 1u);
     return uint16_t(u >> 16);
 }
-__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 // C[M x N] = alpha * op(A)[M x K] op(B)[K x N] (+ bias[n]) (ReLU), A(i, k) = A[i * sa_m + k * sa_k],
 // B(k, j) = B[k * sb_k + j * sb_n]; FP32 result to C (row-major, ldc) and, when Cb is given, a bf16 copy.
