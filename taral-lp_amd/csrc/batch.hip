@@ -113,25 +113,21 @@ __global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict
     epi.load(inst);
     if (k1 - k0 > CAP) {
         T s = 0;
-        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * vi[uint32_t(idx[k]) & kMask];
+        gather_rounds<kWG, 8>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
+                              [&](int, int, T x) { s += x; });
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
         if (tid == 0) epi(r0, (red[0] + red[1]) + (red[2] + red[3]));
         return;
     }
-#pragma unroll
-    for (int u = 0; u < CAP / kWG; ++u) {
-        const int k = k0 + u * kWG + tid;
-        if (k < k1) {
-            if constexpr (SORTED) {
-                const uint32_t p = uint32_t(idx[k]);
-                sh[p >> kBatchColBits] = val[k] * vi[p & kMask];
-            } else {
-                sh[k - k0] = val[k] * vi[idx[k]];
-            }
-        }
-    }
+    gather_rounds<kWG, CAP / kWG>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
+                                  [&](int k, int c, T x) {
+                                      if constexpr (SORTED)
+                                          sh[uint32_t(c) >> kBatchColBits] = x;
+                                      else
+                                          sh[k - k0] = x;
+                                  });
     __syncthreads();
     if constexpr (SORTED) {  // as hipcommon.hpp's k_sorted
         block_row_sums<T>(P, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });

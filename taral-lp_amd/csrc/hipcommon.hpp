@@ -46,6 +46,41 @@ struct EpiStore {  // y = M v
     __device__ void operator()(int r, T s) const { y[r] = s; }
 };
 
+// Gathers of nonzeros [k0, k1), U per thread per round (nonzero base + u * WG + tid): each lane first loads its U
+// index words and values, then its U gathered entries, all at indices clamped into [k0, k1) so that no load sits
+// behind a bounds test, and only then hands the in-range products to use(k, word, product) in the order of k.
+// With a bounds test around each load the compiler waits for every gather before issuing the next one (one load
+// chain in flight per wave); this way a wave keeps 2U and then U loads in flight.
+template <int WG, int U, class T, class W, class Col, class Use>
+__device__ __forceinline__ void gather_rounds(int k0, int k1, const W* __restrict__ w, const T* __restrict__ val,
+                                              const T* __restrict__ v, Col col, Use use) {
+    if (k1 <= k0) return;
+    const int tid = int(threadIdx.x);
+    for (int base = k0; base < k1; base += U * WG) {
+        W p[U];
+        T a[U], g[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = min(base + u * WG + tid, k1 - 1);
+            p[u] = w[k];
+            a[u] = val[k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) g[u] = v[col(p[u])];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + u * WG + tid < k1) use(base + u * WG + tid, p[u], a[u] * g[u]);
+    }
+}
+
+struct ColPlain {
+    __device__ int operator()(int c) const { return c; }
+};
+struct ColPacked {
+    uint32_t mask;
+    __device__ uint32_t operator()(uint32_t p) const { return p & mask; }
+};
+
 // Row-block SpMV. Workgroups [0, nblk) take the row blocks; a row longer than kCap nonzeros forms a block of its
 // own, which is skipped here: workgroups [nblk, nblk + nch) each take one kCap-nonzero chunk of such a row
 // (chunk c: row, first and end nonzero in chunk[3c..3c+2]) and write its sum to lpart[c]; k_long_rows then adds
@@ -62,7 +97,7 @@ __global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restri
         const int c = int(blockIdx.x) - nblk;
         const int k0 = chunk[3 * c + 1], k1 = chunk[3 * c + 2];
         T s = 0;
-        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * v[idx[k]];
+        gather_rounds<kWG, 8>(k0, k1, idx, val, v, ColPlain{}, [&](int, int, T x) { s += x; });
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
@@ -74,11 +109,7 @@ __global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restri
     const int k0 = ptr[r0], k1 = ptr[r1];
     if (k1 - k0 > kCap) return;  // a long row: its chunks and k_long_rows handle it
     epi.load();
-#pragma unroll
-    for (int u = 0; u < kCap / kWG; ++u) {
-        const int k = k0 + u * kWG + tid;
-        if (k < k1) sh[k - k0] = val[k] * v[idx[k]];
-    }
+    gather_rounds<kWG, kCap / kWG>(k0, k1, idx, val, v, ColPlain{}, [&](int k, int, T x) { sh[k - k0] = x; });
     __syncthreads();
     const int nr = r1 - r0;
     int lg = 6;  // lanes per row = 2^lg: the widest group with nr groups fitting the workgroup
@@ -106,15 +137,16 @@ __global__ __launch_bounds__(kWG) void k_long_rows(int nlr, const int* __restric
     }
 }
 
-// Row sums of a staged block (products of rows [r0, r1) in CSR order at sh[ptr[r] - k0 ...]): blocks of at most
-// kWG rows give each row a power-of-two lane group (xor shuffles inside a wave), larger blocks one thread per row.
-template <class T, class Out>
+// Row sums of a staged block (products of rows [r0, r1) in CSR order at sh[ptr[r] - k0 ...]) by a workgroup of
+// WG threads: blocks of at most WG rows give each row a power-of-two lane group (xor shuffles inside a wave),
+// larger blocks one thread per row.
+template <class T, int WG = kWG, class Out>
 __device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int r0, int r1, int k0, const T* sh,
                                                Out&& out) {
     const int tid = int(threadIdx.x), nr = r1 - r0;
-    if (nr <= kWG) {
+    if (nr <= WG) {
         int lg = 6;
-        while (lg > 0 && (nr << lg) > kWG) --lg;
+        while (lg > 0 && (nr << lg) > WG) --lg;
         const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
         T s = 0;
         if (row < nr) {
@@ -125,11 +157,22 @@ __device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int 
         if (row < nr && lane == 0) out(r0 + row, s);
         return;
     }
-    for (int r = r0 + tid; r < r1; r += kWG) {
-        const int a = ptr[r] - k0, e = ptr[r + 1] - k0;
-        T s = 0;
-        for (int j = a; j < e; ++j) s += sh[j];
-        out(r, s);
+    for (int rb = r0; rb < r1; rb += 4 * WG) {  // rows rb + q * WG + tid; the row pointers loaded up front
+        int a[4], e[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = min(rb + q * WG + tid, r1 - 1);
+            a[q] = ptr[r] - k0, e[q] = ptr[r + 1] - k0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = rb + q * WG + tid;
+            if (r < r1) {
+                T s = 0;
+                for (int j = a[q]; j < e[q]; ++j) s += sh[j];
+                out(r, s);
+            }
+        }
     }
 }
 
@@ -151,7 +194,7 @@ __global__ __launch_bounds__(kWG) void k_sorted(int nblk, const int* __restrict_
         const int c = int(blockIdx.x) - nblk;
         const int k0 = chunk[3 * c + 1], k1 = chunk[3 * c + 2];
         T s = 0;
-        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * v[pk[k] & mask];
+        gather_rounds<kWG, 8>(k0, k1, pk, val, v, ColPacked{mask}, [&](int, uint32_t, T x) { s += x; });
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
@@ -163,30 +206,36 @@ __global__ __launch_bounds__(kWG) void k_sorted(int nblk, const int* __restrict_
     const int k0 = ptr[r0], k1 = ptr[r1];
     if (k1 - k0 > kCap && r1 - r0 == 1) return;  // a long row: its chunks and k_long_rows handle it
     epi.load();
-    for (int base = k0; base < k1; base += 8 * kWG) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int k = base + u * kWG + tid;
-            if (k < k1) {
-                const uint32_t p = pk[k];
-                sh[p >> cbits] = val[k] * v[p & mask];
-            }
-        }
-    }
+    gather_rounds<kWG, 8>(k0, k1, pk, val, v, ColPacked{mask}, [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
     __syncthreads();
     block_row_sums<T>(ptr, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });
 }
 
-// Sorted slab layout: workgroup b takes row block b / 8 of slab b % 8 (columns local to the slab, gathered from
-// v + sstart[slab]) and writes the slab's partial row sums part[slab * rows + row]; same staging as k_sorted.
-template <class T, int SB>
-__global__ __launch_bounds__(kWG) void k_sorted_slab(int rows, int nslab, const int* __restrict__ snblk,
-                                                     const int* __restrict__ sboff, const int* __restrict__ blk_all,
-                                                     const int* __restrict__ ptr_all, const uint32_t* __restrict__ pk,
-                                                     const T* __restrict__ val, const T* __restrict__ v,
-                                                     const int* __restrict__ sstart, int cbits, T* __restrict__ part) {
+// Pairwise sum of N values (N a power of two): ((r0 + r1) + (r2 + r3)) + ...
+template <int N, class T>
+__device__ __forceinline__ T tree_sum(const T* r) {
+    if constexpr (N == 1)
+        return r[0];
+    else
+        return tree_sum<N / 2>(r) + tree_sum<N / 2>(r + N / 2);
+}
+
+// Threads per workgroup of the sorted slab kernel: one per 16 staged nonzeros, between kWG and 1,024. Measured at
+// config 1 (FP32, 16,384-nonzero blocks): A x 209 -> 174 us from 256 to 1,024 threads; two such workgroups fill a
+// CU's 160 KB of LDS and its 32 wave slots, and the row-sum phase after the gathers is split over 4x the threads.
+constexpr int sorted_slab_wg(int sb) { return sb / 16 < kWG ? kWG : (sb / 16 > 1024 ? 1024 : sb / 16); }
+
+// Sorted slab layout: workgroup b takes row block b / nslab of slab b % nslab (columns local to the slab,
+// gathered from v + sstart[slab]) and writes the slab's partial row sums part[slab * rows + row]; same staging as
+// k_sorted, with WG threads.
+template <class T, int SB, int WG>
+__global__ __launch_bounds__(WG) void k_sorted_slab(int rows, int nslab, const int* __restrict__ snblk,
+                                                    const int* __restrict__ sboff, const int* __restrict__ blk_all,
+                                                    const int* __restrict__ ptr_all, const uint32_t* __restrict__ pk,
+                                                    const T* __restrict__ val, const T* __restrict__ v,
+                                                    const int* __restrict__ sstart, int cbits, T* __restrict__ part) {
     __shared__ T sh[SB];
-    __shared__ T red[kWG / 64];
+    __shared__ T red[WG / 64];
     const int q = int(blockIdx.x) % nslab, lb = int(blockIdx.x) / nslab;
     if (lb >= snblk[q]) return;
     const int* blk = blk_all + sboff[q];
@@ -199,25 +248,16 @@ __global__ __launch_bounds__(kWG) void k_sorted_slab(int rows, int nslab, const 
     const int k0 = ptr[r0], k1 = ptr[r1];
     if (k1 - k0 > kCap && r1 - r0 == 1) {  // a long row of this slab: strided sums and a workgroup reduction
         T s = 0;
-        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * vq[pk[k] & mask];
+        gather_rounds<WG, 8>(k0, k1, pk, val, vq, ColPacked{mask}, [&](int, uint32_t, T x) { s += x; });
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
-        if (tid == 0) y[r0] = (red[0] + red[1]) + (red[2] + red[3]);
+        if (tid == 0) y[r0] = tree_sum<WG / 64>(red);
         return;
     }
-    for (int base = k0; base < k1; base += 8 * kWG) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int k = base + u * kWG + tid;
-            if (k < k1) {
-                const uint32_t p = pk[k];
-                sh[p >> cbits] = val[k] * vq[p & mask];
-            }
-        }
-    }
+    gather_rounds<WG, 8>(k0, k1, pk, val, vq, ColPacked{mask}, [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
     __syncthreads();
-    block_row_sums<T>(ptr, r0, r1, k0, sh, [&](int r, T s) { y[r] = s; });
+    block_row_sums<T, WG>(ptr, r0, r1, k0, sh, [&](int r, T s) { y[r] = s; });
 }
 
 // Slab layout, first kernel: workgroup b handles row block b / 8 of slab b % 8 (slab = XCD) and writes the
@@ -240,18 +280,14 @@ __global__ __launch_bounds__(kWG) void k_slab_partial(int rows, int nslab, const
     const int k0 = ptr[r0], k1 = ptr[r1];
     if (k1 - k0 > kCap) {
         T s = 0;
-        for (int k = k0 + tid; k < k1; k += kWG) s += val[k] * v[idx[k]];
+        gather_rounds<kWG, 8>(k0, k1, idx, val, v, ColPlain{}, [&](int, int, T x) { s += x; });
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
         if (tid == 0) y[r0] = (red[0] + red[1]) + (red[2] + red[3]);
         return;
     }
-#pragma unroll
-    for (int u = 0; u < kCap / kWG; ++u) {
-        const int k = k0 + u * kWG + tid;
-        if (k < k1) sh[k - k0] = val[k] * v[idx[k]];
-    }
+    gather_rounds<kWG, kCap / kWG>(k0, k1, idx, val, v, ColPlain{}, [&](int k, int, T x) { sh[k - k0] = x; });
     __syncthreads();
     const int nr = r1 - r0;
     int lg = 6;
@@ -462,10 +498,12 @@ int slab_count(int cols) {
     return k;
 }
 
-// Largest sorted blocks (32 KB of FP32 products in LDS; measured at config 1: 8,192 beats 4,096 and 2,048 and
-// 16,384 loses to occupancy). Plain layout: 13 position bits for columns < 2^19; slabs: 13 or 14 bits.
+// Largest sorted blocks. Plain layout (256 threads): 8,192 (32 KB of FP32 products in LDS; measured at config 1
+// against 4,096 and 2,048), 13 position bits above 19 column bits. Slabs: 16,384 when a slab spans at most 2^18
+// columns (14 position bits; 64 KB of FP32 products, 1,024 threads: 174 us against 209 us for 8,192 blocks at
+// config 1), else 8,192.
 constexpr int kSortPlain = 8192;
-constexpr int kSortSlab = 8192;
+constexpr int kSortSlab = 16384;
 constexpr int kSortMinBlocks = 1024;  // the block size shrinks (down to kCap) until there are this many blocks
 
 int sorted_block_size(int64_t nnz, int max_sb, int value_bytes) {
@@ -575,7 +613,9 @@ void sort_blocks(const std::vector<int32_t>& blk, const int32_t* ptr, const int3
     int64_t width = 0;
     for (int q = 0; q < NS; ++q) width = std::max<int64_t>(width, L.sstart[size_t(q) + 1] - L.sstart[size_t(q)]);
     L.sorted = want_sorted && width <= (1 << 19);
-    if (L.sorted) L.cbits = width <= (1 << 18) ? 18 : 19, L.sb = sorted_block_size(nnz, kSortSlab, value_bytes);
+    if (L.sorted)
+        L.cbits = width <= (1 << 18) ? 18 : 19,
+        L.sb = sorted_block_size(nnz, L.cbits == 18 ? kSortSlab : kSortSlab / 2, value_bytes);
     std::vector<int32_t> local, pk;
     std::vector<int64_t> perm2;
     if (L.sorted) {
@@ -658,17 +698,23 @@ DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, int cols, const int32_t
 template <class T, int SB, class Epi>
 void launch_sorted(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
     if (a.slab) {
+        constexpr int WG = sorted_slab_wg(SB);
         if (a.maxblk > 0)
-            k_sorted_slab<T, SB><<<a.nslab * a.maxblk, kWG, 0, st>>>(a.rows, a.nslab, a.snblk, a.sboff, a.blk, a.ptr,
-                                                                   reinterpret_cast<const uint32_t*>(a.idx), a.val, v,
-                                                                   a.sstart, a.cbits, a.part);
+            k_sorted_slab<T, SB, WG><<<a.nslab * a.maxblk, WG, 0, st>>>(a.rows, a.nslab, a.snblk, a.sboff, a.blk,
+                                                                      a.ptr, reinterpret_cast<const uint32_t*>(a.idx),
+                                                                      a.val, v, a.sstart, a.cbits, a.part);
         k_combine_epi<T, Epi><<<grid_for(a.rows), kWG, 0, st>>>(a.rows, a.nslab, a.part, epi);
         return;
     }
-    if (a.nblk == 0) return;
-    k_sorted<T, SB, Epi><<<a.nblk + a.nch, kWG, 0, st>>>(a.nblk, a.blk, a.ptr, reinterpret_cast<const uint32_t*>(a.idx),
-                                                        a.val, v, a.cbits, epi, a.chunk, a.lpart);
-    if (a.nlr) k_long_rows<T, Epi><<<(a.nlr + kWG - 1) / kWG, kWG, 0, st>>>(a.nlr, a.lrow, a.lpart, epi);
+    if constexpr (SB > kSortPlain) {
+        throw std::logic_error("sorted plain layout with blocks above 8,192 nonzeros");
+    } else {
+        if (a.nblk == 0) return;
+        k_sorted<T, SB, Epi><<<a.nblk + a.nch, kWG, 0, st>>>(a.nblk, a.blk, a.ptr,
+                                                            reinterpret_cast<const uint32_t*>(a.idx), a.val, v,
+                                                            a.cbits, epi, a.chunk, a.lpart);
+        if (a.nlr) k_long_rows<T, Epi><<<(a.nlr + kWG - 1) / kWG, kWG, 0, st>>>(a.nlr, a.lrow, a.lpart, epi);
+    }
 }
 
 template <class T, class Epi>
@@ -678,7 +724,9 @@ void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st)
         switch (a.sb) {
             case 2048: launch_sorted<T, 2048>(a, v, epi, st); return;
             case 4096: launch_sorted<T, 4096>(a, v, epi, st); return;
-            default: launch_sorted<T, 8192>(a, v, epi, st); return;
+            case 8192: launch_sorted<T, 8192>(a, v, epi, st); return;
+            case 16384: launch_sorted<T, 16384>(a, v, epi, st); return;
+            default: throw std::logic_error("unsupported sorted block size");
         }
     }
     if (a.slab) {
