@@ -254,7 +254,7 @@ def run_batch(a, ffi, gen, desc, lps):
     for md in lps:
         r_, c_ = halpern_kernel_bytes(md.m, md.n, md.nnz, 4)
         b_it += r_ + c_
-    ti = {1024: 0, 2048: 1, 4096: 2}.get(info["cap"])
+    ti = {1024: 0, 2048: 1, 4096: 2, 8192: 3}.get(info["cap"])
     it_ms = info["tune_ms"][ti] / 24 if ti is not None and info["tune_ms"][ti] > 0 else 0.0
     achieved = b_it / (it_ms * 1e-3) / 1e9 if it_ms else 0.0
     roofline = dict(bound="hbm", kernel="one batched iteration (row + column launch, all instances live)",
@@ -344,7 +344,7 @@ def run_scenarios(a, ffi, gen, desc, fam):
     for _ in range(bs):
         r_, c_ = halpern_kernel_bytes(fam.base.m, fam.base.n, fam.base.nnz, 4)
         b_it += r_ + c_
-    ti = {1024: 0, 2048: 1, 4096: 2}.get(info["cap"])
+    ti = {1024: 0, 2048: 1, 4096: 2, 8192: 3}.get(info["cap"])
     it_ms = info["tune_ms"][ti] / 24 if ti is not None and info["tune_ms"][ti] > 0 else 0.0
     achieved = b_it / (it_ms * 1e-3) / 1e9 if it_ms else 0.0
     out = {

@@ -246,14 +246,15 @@ int tpdhg_recheck(const tpdhg_model* model, const double* x, const double* y, tp
 typedef struct tpdhg_batch tpdhg_batch;
 typedef struct {
     int32_t count, unique_matrices;
-    int32_t cap;              /* nonzeros staged per row block (1024, 2048 or 4096) */
+    int32_t cap;              /* nonzeros staged per row block (1024, 2048, 4096 or 8192) */
     int64_t nnz_total;        /* over all instances */
     int64_t nnz_stored;       /* held on the device (shared matrices once) */
     int64_t device_mem_bytes;
     double setup_s;           /* scaling, upload and stage-size autotuning */
-    float tune_ms[3];         /* autotune timings for 1024 / 2048 / 4096 (0 when cap was given) */
+    float tune_ms[4];         /* autotune timings (ms, 24 iterations) for 1024 / 2048 / 4096 / 8192; 0 when cap
+                                 was given */
 } tpdhg_batch_info;
-/* opt->halpern must be 1. cap: 0 = pick the row-block stage by timing, else 1024, 2048 or 4096. The batch keeps
+/* opt->halpern must be 1. cap: 0 = pick the row-block stage by timing, else 1024, 2048, 4096 or 8192. The batch keeps
  * copies of the models. */
 int tpdhg_batch_create(const tpdhg_model* const* models, int32_t count, const tpdhg_options* opt, int32_t cap,
                        tpdhg_batch** out);

@@ -93,14 +93,14 @@ struct BEpiHCol {
 // block the nonzeros are in column order with their row-order position packed above the column; otherwise the
 // k_rowblocks scheme. A row longer than CAP is a block of its own, summed strided by the workgroup. With a list,
 // workgroup b takes block list[b] (nblk = list length), so a launch covers only the instances still iterating.
-template <class T, int CAP, bool SORTED, class Epi>
-__global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict__ list, const int* __restrict__ blk,
+template <class T, int CAP, bool SORTED, class Epi, int WG = stage_wg(CAP)>
+__global__ __launch_bounds__(WG) void k_bblocks(int nblk, const int* __restrict__ list, const int* __restrict__ blk,
                                                  const int* __restrict__ pbase, const int* __restrict__ ptr,
                                                  const int* __restrict__ idx, const T* __restrict__ val,
                                                  const int* __restrict__ voff, const T* __restrict__ v,
                                                  const uint8_t* __restrict__ on, Epi epi) {
     __shared__ T sh[CAP];
-    __shared__ T red[kWG / 64];
+    __shared__ T red[WG / 64];
     const int lb = xcd_block(int(blockIdx.x), nblk);
     const int b = list ? list[lb] : lb;  // list: the live instances' blocks, in instance order
     const int inst = blk[3 * b], r0 = blk[3 * b + 1], r1 = blk[3 * b + 2];
@@ -113,29 +113,29 @@ __global__ __launch_bounds__(kWG) void k_bblocks(int nblk, const int* __restrict
     epi.load(inst);
     if (k1 - k0 > CAP) {
         T s = 0;
-        gather_rounds<kWG, 8>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
-                              [&](int, int, T x) { s += x; });
+        gather_rounds<WG, 8>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
+                             [&](int, int, T x) { s += x; });
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
-        if (tid == 0) epi(r0, (red[0] + red[1]) + (red[2] + red[3]));
+        if (tid == 0) epi(r0, tree_sum<WG / 64>(red));
         return;
     }
-    gather_rounds<kWG, CAP / kWG>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
-                                  [&](int k, int c, T x) {
-                                      if constexpr (SORTED)
-                                          sh[uint32_t(c) >> kBatchColBits] = x;
-                                      else
-                                          sh[k - k0] = x;
-                                  });
+    gather_rounds<WG, CAP / WG>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
+                                [&](int k, int c, T x) {
+                                    if constexpr (SORTED)
+                                        sh[uint32_t(c) >> kBatchColBits] = x;
+                                    else
+                                        sh[k - k0] = x;
+                                });
     __syncthreads();
     if constexpr (SORTED) {  // as hipcommon.hpp's k_sorted
-        block_row_sums<T>(P, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });
+        block_row_sums<T, WG>(P, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });
         return;
     }
     const int nr = r1 - r0;
     int lg = 6;
-    while (lg > 0 && (nr << lg) > kWG) --lg;
+    while (lg > 0 && (nr << lg) > WG) --lg;
     const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
     T s = 0;
     if (row < nr) {
@@ -349,8 +349,8 @@ public:
         if (B_ <= 0) throw std::runtime_error("empty batch");
         if (!(o.rho >= 1 && o.rho < 2)) throw std::runtime_error("reflection coefficient rho must be in [1, 2)");
         if (o.eval_freq < 1) throw std::runtime_error("eval_freq must be >= 1");
-        if (o.cap != 0 && o.cap != 1024 && o.cap != 2048 && o.cap != 4096)
-            throw std::runtime_error("batch row-block stage must be 0 (autotune), 1024, 2048 or 4096 nonzeros");
+        if (o.cap != 0 && o.cap != 1024 && o.cap != 2048 && o.cap != 4096 && o.cap != 8192)
+            throw std::runtime_error("batch row-block stage must be 0 (autotune), 1024, 2048, 4096 or 8192 nonzeros");
         size_t free0 = 0, free1 = 0, total = 0;
         HIP_OK(hipMemGetInfo(&free0, &total));
         name_ = device_name();
@@ -814,7 +814,7 @@ private:
         HIP_OK(hipEventCreate(&e1));
         float best = 0;
         int best_cap = kCap;
-        for (int cap : {1024, 2048, 4096}) {
+        for (int cap : {1024, 2048, 4096, 8192}) {
             set_cap(cap);
             upload_act(all);
             ensure_lists(all);
@@ -856,18 +856,20 @@ private:
         const int nb = list ? nlist : side.nblk;
         if (nb == 0) return;
 #define TPDHG_BB(CAPV, SRT)                                                                                      \
-    k_bblocks<U, CAPV, SRT, Epi><<<nb, kWG, 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val, \
-                                                      side.voff, v, on, epi)
+    k_bblocks<U, CAPV, SRT, Epi><<<nb, stage_wg(CAPV), 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr,        \
+                                                                 side.idx, side.val, side.voff, v, on, epi)
         if (sorted_) {
             switch (cap_) {
                 case 1024: TPDHG_BB(1024, true); break;
                 case 4096: TPDHG_BB(4096, true); break;
+                case 8192: TPDHG_BB(8192, true); break;
                 default: TPDHG_BB(2048, true);
             }
         } else {
             switch (cap_) {
                 case 1024: TPDHG_BB(1024, false); break;
                 case 4096: TPDHG_BB(4096, false); break;
+                case 8192: TPDHG_BB(8192, false); break;
                 default: TPDHG_BB(2048, false);
             }
         }

@@ -50,7 +50,7 @@ public:
     virtual int64_t nnz_total() const = 0;    // sum over instances
     virtual int unique_matrices() const = 0;
     virtual int cap() const = 0;  // row-block stage size in use
-    // Autotune timings (ms for 28 iterations, stage sizes 1024, 2048, 4096; empty when the size was given).
+    // Autotune timings (ms for 24 iterations, stage sizes 1024, 2048, 4096, 8192; empty when the size was given).
     virtual std::vector<float> tune_ms() const = 0;
 };
 
@@ -59,7 +59,7 @@ struct BatchOptions {
     int eval_freq = 64;
     bool use_graph = true;
     double rho = 1.9;
-    int cap = 0;  // nonzeros staged per row block: 1024, 2048 or 4096; 0 = pick by timing (autotune)
+    int cap = 0;  // nonzeros staged per row block: 1024, 2048, 4096 or 8192; 0 = pick by timing (autotune)
 };
 
 // Instances are the scaled problems; instances whose scaled A (and so A^T) is identical share the device copy.

@@ -454,7 +454,8 @@ int tpdhg_batch_create(const tpdhg_model* const* models, int32_t count, const tp
     if (o.eval_freq < 1) o.eval_freq = 1;
     if (int rc = check_options(o)) return rc;
     if (!o.halpern) return fail(TPDHG_ERR_ARG, "batches run the fast path: set halpern = 1");
-    if (cap != 0 && cap != 1024 && cap != 2048 && cap != 4096) return fail(TPDHG_ERR_ARG, "cap must be 0, 1024, 2048 or 4096");
+    if (cap != 0 && cap != 1024 && cap != 2048 && cap != 4096 && cap != 8192)
+        return fail(TPDHG_ERR_ARG, "cap must be 0, 1024, 2048, 4096 or 8192");
     std::vector<const tpdhg::LP*> lps;
     for (int32_t i = 0; i < count; ++i) {
         if (!models[i]) return fail(TPDHG_ERR_ARG, "null model in batch");
@@ -478,7 +479,7 @@ int tpdhg_batch_get_info(const tpdhg_batch* b, tpdhg_batch_info* info) {
     info->nnz_total = d.nnz_total(), info->nnz_stored = d.nnz_stored();
     info->device_mem_bytes = int64_t(d.mem_used()), info->setup_s = b->s->setup_s();
     const std::vector<float> t = d.tune_ms();
-    for (size_t k = 0; k < t.size() && k < 3; ++k) info->tune_ms[k] = t[k];
+    for (size_t k = 0; k < t.size() && k < 4; ++k) info->tune_ms[k] = t[k];
     return TPDHG_OK;
 }
 

@@ -176,41 +176,6 @@ __device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int 
     }
 }
 
-// Sorted layout. Inside each row block (at most SB nonzeros, rows of at most kCap) the nonzeros are stored in
-// ascending column order, each with its row-order position in the block packed above the column:
-// pk = (pos << cbits) | col. A wave's 64 lanes then gather from nearby columns, so several lanes share a cache
-// line and the L2 sees fewer requests than nonzeros; each product goes to LDS at its row-order position and the
-// rows are summed there (block_row_sums). Rows longer than kCap are chunked as in k_rowblocks.
-template <class T, int SB, class Epi>
-__global__ __launch_bounds__(kWG) void k_sorted(int nblk, const int* __restrict__ blk, const int* __restrict__ ptr,
-                                                const uint32_t* __restrict__ pk, const T* __restrict__ val,
-                                                const T* __restrict__ v, int cbits, Epi epi,
-                                                const int* __restrict__ chunk, T* __restrict__ lpart) {
-    __shared__ T sh[SB];
-    __shared__ T red[kWG / 64];
-    const int tid = int(threadIdx.x);
-    const uint32_t mask = (1u << cbits) - 1u;
-    if (int(blockIdx.x) >= nblk) {  // a chunk of a long row
-        const int c = int(blockIdx.x) - nblk;
-        const int k0 = chunk[3 * c + 1], k1 = chunk[3 * c + 2];
-        T s = 0;
-        gather_rounds<kWG, 8>(k0, k1, pk, val, v, ColPacked{mask}, [&](int, uint32_t, T x) { s += x; });
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if ((tid & 63) == 0) red[tid >> 6] = s;
-        __syncthreads();
-        if (tid == 0) lpart[c] = (red[0] + red[1]) + (red[2] + red[3]);
-        return;
-    }
-    const int b = xcd_block(int(blockIdx.x), nblk);
-    const int r0 = blk[b], r1 = blk[b + 1];
-    const int k0 = ptr[r0], k1 = ptr[r1];
-    if (k1 - k0 > kCap && r1 - r0 == 1) return;  // a long row: its chunks and k_long_rows handle it
-    epi.load();
-    gather_rounds<kWG, 8>(k0, k1, pk, val, v, ColPacked{mask}, [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
-    __syncthreads();
-    block_row_sums<T>(ptr, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });
-}
-
 // Pairwise sum of N values (N a power of two): ((r0 + r1) + (r2 + r3)) + ...
 template <int N, class T>
 __device__ __forceinline__ T tree_sum(const T* r) {
@@ -220,10 +185,47 @@ __device__ __forceinline__ T tree_sum(const T* r) {
         return tree_sum<N / 2>(r) + tree_sum<N / 2>(r + N / 2);
 }
 
-// Threads per workgroup of the sorted slab kernel: one per 16 staged nonzeros, between kWG and 1,024. Measured at
-// config 1 (FP32, 16,384-nonzero blocks): A x 209 -> 174 us from 256 to 1,024 threads; two such workgroups fill a
-// CU's 160 KB of LDS and its 32 wave slots, and the row-sum phase after the gathers is split over 4x the threads.
-constexpr int sorted_slab_wg(int sb) { return sb / 16 < kWG ? kWG : (sb / 16 > 1024 ? 1024 : sb / 16); }
+// Threads per workgroup of the kernels that stage SB nonzeros in LDS (sorted, sorted slab, batched blocks): one
+// per 16 staged nonzeros, between kWG and 1,024. Measured at config 1 (FP32 slabs, 16,384-nonzero blocks): A x
+// 209 -> 174 us from 256 to 1,024 threads; two such workgroups fill a CU's 160 KB of LDS and its 32 wave slots, and
+// the row-sum phase after the gathers is split over 4x the threads. Equal SB gives equal WG in every kernel, so
+// equal blocks are summed in the same order everywhere.
+constexpr int stage_wg(int sb) { return sb / 16 < kWG ? kWG : (sb / 16 > 1024 ? 1024 : sb / 16); }
+
+// Sorted layout. Inside each row block (at most SB nonzeros, rows of at most kCap) the nonzeros are stored in
+// ascending column order, each with its row-order position in the block packed above the column:
+// pk = (pos << cbits) | col. A wave's 64 lanes then gather from nearby columns, so several lanes share a cache
+// line and the L2 sees fewer requests than nonzeros; each product goes to LDS at its row-order position and the
+// rows are summed there (block_row_sums). Rows longer than kCap are chunked as in k_rowblocks.
+template <class T, int SB, int WG, class Epi>
+__global__ __launch_bounds__(WG) void k_sorted(int nblk, const int* __restrict__ blk, const int* __restrict__ ptr,
+                                               const uint32_t* __restrict__ pk, const T* __restrict__ val,
+                                               const T* __restrict__ v, int cbits, Epi epi,
+                                               const int* __restrict__ chunk, T* __restrict__ lpart) {
+    __shared__ T sh[SB];
+    __shared__ T red[WG / 64];
+    const int tid = int(threadIdx.x);
+    const uint32_t mask = (1u << cbits) - 1u;
+    if (int(blockIdx.x) >= nblk) {  // a chunk of a long row
+        const int c = int(blockIdx.x) - nblk;
+        const int k0 = chunk[3 * c + 1], k1 = chunk[3 * c + 2];
+        T s = 0;
+        gather_rounds<WG, kCap / WG>(k0, k1, pk, val, v, ColPacked{mask}, [&](int, uint32_t, T x) { s += x; });
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) lpart[c] = tree_sum<WG / 64>(red);
+        return;
+    }
+    const int b = xcd_block(int(blockIdx.x), nblk);
+    const int r0 = blk[b], r1 = blk[b + 1];
+    const int k0 = ptr[r0], k1 = ptr[r1];
+    if (k1 - k0 > kCap && r1 - r0 == 1) return;  // a long row: its chunks and k_long_rows handle it
+    epi.load();
+    gather_rounds<WG, 8>(k0, k1, pk, val, v, ColPacked{mask}, [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
+    __syncthreads();
+    block_row_sums<T, WG>(ptr, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });
+}
 
 // Sorted slab layout: workgroup b takes row block b / nslab of slab b % nslab (columns local to the slab,
 // gathered from v + sstart[slab]) and writes the slab's partial row sums part[slab * rows + row]; same staging as
@@ -698,7 +700,7 @@ DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, int cols, const int32_t
 template <class T, int SB, class Epi>
 void launch_sorted(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
     if (a.slab) {
-        constexpr int WG = sorted_slab_wg(SB);
+        constexpr int WG = stage_wg(SB);
         if (a.maxblk > 0)
             k_sorted_slab<T, SB, WG><<<a.nslab * a.maxblk, WG, 0, st>>>(a.rows, a.nslab, a.snblk, a.sboff, a.blk,
                                                                       a.ptr, reinterpret_cast<const uint32_t*>(a.idx),
@@ -710,9 +712,10 @@ void launch_sorted(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t s
         throw std::logic_error("sorted plain layout with blocks above 8,192 nonzeros");
     } else {
         if (a.nblk == 0) return;
-        k_sorted<T, SB, Epi><<<a.nblk + a.nch, kWG, 0, st>>>(a.nblk, a.blk, a.ptr,
-                                                            reinterpret_cast<const uint32_t*>(a.idx), a.val, v,
-                                                            a.cbits, epi, a.chunk, a.lpart);
+        constexpr int WG = stage_wg(SB);
+        k_sorted<T, SB, WG, Epi><<<a.nblk + a.nch, WG, 0, st>>>(a.nblk, a.blk, a.ptr,
+                                                               reinterpret_cast<const uint32_t*>(a.idx), a.val, v,
+                                                               a.cbits, epi, a.chunk, a.lpart);
         if (a.nlr) k_long_rows<T, Epi><<<(a.nlr + kWG - 1) / kWG, kWG, 0, st>>>(a.nlr, a.lrow, a.lpart, epi);
     }
 }

@@ -99,7 +99,7 @@ class RecheckResult(C.Structure):
 class BatchInfo(C.Structure):
     _fields_ = [("count", C.c_int32), ("unique_matrices", C.c_int32), ("cap", C.c_int32),
                 ("nnz_total", C.c_int64), ("nnz_stored", C.c_int64), ("device_mem_bytes", C.c_int64),
-                ("setup_s", C.c_double), ("tune_ms", C.c_float * 3)]
+                ("setup_s", C.c_double), ("tune_ms", C.c_float * 4)]
 
 
 _lib = None

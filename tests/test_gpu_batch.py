@@ -109,7 +109,7 @@ def test_autotuned_stage_and_uneven_sizes():
     batch = ffi.Batch(models, **FAST)
     info = batch.info()
     print("autotune", info)
-    assert info["cap"] in (1024, 2048, 4096) and min(info["tune_ms"]) > 0
+    assert info["cap"] in (1024, 2048, 4096, 8192) and min(info["tune_ms"]) > 0
     for md, b in zip(lps, batch.solve()):
         assert b["status"] == "near_optimal"
         assert max(rel_kkt(md, b["x"], b["y"])) <= 1.0001e-6
