@@ -486,18 +486,17 @@ struct HostLayout {
     int nslab = 0;
 };
 
-// Column slabs of the slab layout: the fewest of 2, 4, 8 that keep one slab's slice of the gathered vector within
-// 1 MiB of FP32 (measured at config 1: A x gathering 2M entries is fastest with 8 slabs, 207 us against 225 with 4;
-// A^T z gathering 1M entries with 4, 205 us against 218 with 8). Workgroup b works on slab b mod nslab, so with
-// fewer than 8 slabs XCDs x and x + nslab share one. TPDHG_SLABS=2|4|8 forces a count.
-int slab_count(int cols) {
+// Column slabs of the slab layout: one per XCD. Measured at config 1 (FP32, sorted slab blocks with batched
+// gathers): A x gathering 2M entries 166 us with 8 slabs against 207 with 4 and 254 with 2; A^T z gathering 1M
+// entries 154 us with 8 against 166 with 4 and 206 with 2 (more slabs cost partial sums and row pointers, fewer
+// slabs give each XCD a larger slice to gather from). Workgroup b works on slab b mod nslab, so with fewer than 8
+// slabs XCDs x and x + nslab share one. TPDHG_SLABS=2|4|8 forces a count.
+int slab_count(int) {
     if (const char* e = std::getenv("TPDHG_SLABS")) {
         const int k = std::atoi(e);
         if (k == 2 || k == 4 || k == 8) return k;
     }
-    int k = 2;
-    while (k < kSlabs && int64_t(cols) * 4 > int64_t(k) << 20) k *= 2;
-    return k;
+    return kSlabs;
 }
 
 // Largest sorted blocks. Plain layout (256 threads): 8,192 (32 KB of FP32 products in LDS; measured at config 1
