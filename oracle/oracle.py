@@ -58,6 +58,7 @@ def lib():
         L.or_solve.argtypes = [C.POINTER(Scaled), C.c_int, C.c_double, C.c_long, C.c_int, C.c_double,
                                C.c_double, _d, _d, _d, _d, C.POINTER(Result)]
         L.or_block.argtypes = [C.POINTER(Scaled), C.c_int, C.c_double, C.c_double, _d, _d, C.c_int] + [_d] * 8
+        L.or_block_mt.argtypes = [C.POINTER(Scaled), C.c_int, C.c_double, C.c_double, _d, _d, C.c_int, C.c_int] + [_d] * 8
         L.or_finish.argtypes = [C.POINTER(Scaled), C.c_int, _i64, _i32, _d, _d, C.c_double, _d, _d, _d, _d,
                                 _d, _d, _d, _d]
         _lib = L
@@ -111,15 +112,16 @@ class OracleLP:
         lib().or_eval(C.byref(self.S), int(fp32), f, *[_p(a) for a in arrs], _p(out))
         return out
 
-    def block(self, tau, sigma, steps, x0=None, z0=None, fp32=False):
-        """start + iterate(steps) of the CPU backend; dict of x, z, xs, zs, Ax, ATz, Axs, ATzs."""
+    def block(self, tau, sigma, steps, x0=None, z0=None, fp32=False, threads=1):
+        """start + iterate(steps) of the CPU backend (its iterations on `threads` threads, partitioned as the
+        reference's Pool); dict of x, z, xs, zs, Ax, ATz, Axs, ATzs."""
         m, n = self.S.m, self.S.n
         out = {k: np.zeros(max(m if k in ("z", "zs", "Ax", "Axs") else n, 1))
                for k in ("x", "z", "xs", "zs", "Ax", "ATz", "Axs", "ATzs")}
         x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         z0 = None if z0 is None else np.ascontiguousarray(z0, dtype=np.float64)
-        lib().or_block(C.byref(self.S), int(fp32), float(tau), float(sigma), _p(x0), _p(z0), int(steps),
-                       *[_p(out[k]) for k in ("x", "z", "xs", "zs", "Ax", "ATz", "Axs", "ATzs")])
+        lib().or_block_mt(C.byref(self.S), int(fp32), float(tau), float(sigma), _p(x0), _p(z0), int(steps),
+                          int(threads), *[_p(out[k]) for k in ("x", "z", "xs", "zs", "Ax", "ATz", "Axs", "ATzs")])
         return {k: v[:(m if k in ("z", "zs", "Ax", "Axs") else n)] for k, v in out.items()}
 
     def solve(self, fp32=False, tol=1e-4, max_iter=2_000_000_000, eval_freq=64, row_rel=0.0,

@@ -3,6 +3,7 @@
 #include "pdhg_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -155,6 +156,26 @@ void or_spmv(int rows, const int32_t* ptr, const int32_t* idx, const double* val
     }
 }
 
+/* Thread boundaries of the CPU backend's matrices (gpu/pdhg.cu:601-614): part[0] = 0, part[nt] = rows, and
+ * part[t] is the first row whose prefix rows + nonzeros reaches t/nt of the total. */
+static void or_partition(int rows, const int32_t* ptr, int nt, int* part) {
+    long total = (long)ptr[rows] + rows;
+    part[0] = 0;
+    for (int t = 1; t < nt; ++t) {
+        long target = total * t / nt;
+        int lo = 0, hi = rows;
+        while (lo < hi) {
+            int mid = lo + (hi - lo) / 2;
+            if ((long)ptr[mid] + mid < target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        part[t] = lo;
+    }
+    part[nt] = rows;
+}
+
 #define OT double
 #define OS _d
 #include "pdhg_oracle_t.inc"
@@ -221,7 +242,7 @@ int or_solve(const or_scaled* P, int fp32, double tol, long max_iter, int eval_f
         or_state_init##SUF(&S, P);                                                                       \
         S.tau = (OR_T)tau, S.sigma = (OR_T)sigma;                                                        \
         or_start##SUF(&S, P, x0, z0);                                                                    \
-        if (steps > 0) or_iterate##SUF(&S, steps);                                                       \
+        if (steps > 0) or_iterate_mt##SUF(&S, steps, threads);                                           \
         const OR_T* src[8] = {S.x, S.z, S.xs, S.zs, S.Ax, S.ATz, S.Axs, S.ATzs};                         \
         double* dst[8] = {x, z, xs, zs, Ax, ATz, Axs, ATzs};                                             \
         for (int q = 0; q < 8; ++q) {                                                                    \
@@ -233,6 +254,13 @@ int or_solve(const or_scaled* P, int fp32, double tol, long max_iter, int eval_f
 
 void or_block(const or_scaled* P, int fp32, double tau, double sigma, const double* x0, const double* z0, int steps,
               double* x, double* z, double* xs, double* zs, double* Ax, double* ATz, double* Axs, double* ATzs) {
+    or_block_mt(P, fp32, tau, sigma, x0, z0, steps, 1, x, z, xs, zs, Ax, ATz, Axs, ATzs);
+}
+
+void or_block_mt(const or_scaled* P, int fp32, double tau, double sigma, const double* x0, const double* z0,
+                 int steps, int threads, double* x, double* z, double* xs, double* zs, double* Ax, double* ATz,
+                 double* Axs, double* ATzs) {
+    if (threads < 1) threads = 1;
     if (fp32) {
 #define OR_T float
         OR_BLOCK_BODY(_f);

@@ -70,6 +70,13 @@ int or_solve(const or_scaled* P, int fp32, double tol, long max_iter, int eval_f
 void or_block(const or_scaled* P, int fp32, double tau, double sigma, const double* x0, const double* z0, int steps,
               double* x, double* z, double* xs, double* zs, double* Ax, double* ATz, double* Axs, double* ATzs);
 
+/* or_block with the iterations on `threads` threads partitioned as the CPU backend's Pool (gpu/pdhg.cu:601-614,
+ * 669-691); the result equals or_block's bit for bit (no phase reduces across rows). Used to time the CPU
+ * baseline on all cores; pinned against or_block. */
+void or_block_mt(const or_scaled* P, int fp32, double tau, double sigma, const double* x0, const double* z0,
+                 int steps, int threads, double* x, double* z, double* xs, double* zs, double* Ax, double* ATz,
+                 double* Axs, double* ATzs);
+
 /* Objective on the ORIGINAL model in long double and max row/bound violation, gpu/pdhg.cu:866-876.
  * x is the scaled point; it is unscaled with Dc here. Pinned by the ledgers' objective and max_row_violation. */
 void or_finish(const or_scaled* P, int n, const int64_t* colptr, const int32_t* rowidx, const double* val,

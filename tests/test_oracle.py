@@ -153,3 +153,25 @@ def test_power_norm_against_svd():
     As = o.vec("Dr")[:, None] * A * o.vec("Dc")[None, :]
     smax = np.linalg.svd(As, compute_uv=False)[0]
     assert o.power_norm(iters=2000) == pytest.approx(smax, rel=1e-5)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_threaded_block_equals_single_thread(fp32):
+    """The CPU baseline's threaded iteration block (the reference Pool's partition, a barrier per phase) against
+    the single-thread block: no phase reduces across rows, so every state vector must agree bit for bit; a wrong
+    partition (rows skipped or done twice) or a missing barrier (a phase reading the other's half-written
+    vector) changes them. Thread counts include uneven splits and more threads than rows."""
+    md = gen.planted_lp(300, 700, 6, seed=5)
+    o = oracle.OracleLP(md)
+    want = o.block(0.4, 0.6, 40, fp32=fp32)
+    for threads in (2, 3, 8):
+        got = o.block(0.4, 0.6, 40, fp32=fp32, threads=threads)
+        for k in want:
+            np.testing.assert_array_equal(got[k], want[k], err_msg=f"{k} with {threads} threads")
+    tiny, _ = _tiny(seed=3, m=5, n=9, density=0.5)
+    ot = oracle.OracleLP(tiny)
+    x0 = np.linspace(0.0, 1.0, tiny.n)
+    want = ot.block(0.3, 0.5, 12, x0=x0, fp32=fp32)
+    got = ot.block(0.3, 0.5, 12, x0=x0, fp32=fp32, threads=16)
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
