@@ -19,7 +19,7 @@ namespace {
 constexpr int kWG = 256;    // threads per workgroup: 4 waves of 64
 constexpr int kCap = 2048;  // nonzeros staged per row block (8 per thread)
 constexpr int kMaxRedBlocks = 1024;
-constexpr int kSlabsK = 8;  // column slabs of the slab layout: one per XCD
+constexpr int kSlabsK = 8;  // XCDs: the slab layout uses a multiple of this many column slabs
 
 #define HIP_OK(call)                                                                                      \
     do {                                                                                                  \
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kWG) void k_slab_partial(int rows, int nslab, const
     if (row < nr && lane == 0) y[r0 + row] = s;
 }
 
-// Slab layout, second kernel: the eight partial sums of each row in slab order, then the epilogue.
+// Slab layout, second kernel: the nslab partial sums of each row in slab order, then the epilogue.
 template <class T, class Epi>
 __global__ __launch_bounds__(kWG) void k_combine_epi(int rows, int nslab, const T* __restrict__ part, Epi epi) {
     epi.load();
@@ -449,11 +449,11 @@ struct Mem {  // device allocations owned by one backend
 
 // One CSR on the device in one of two layouts.
 //  plain: row blocks over the whole matrix (k_rowblocks).
-//  slab:  for a gathered vector of at least kSlabMinCols entries (half an XCD's L2 in FP32), the columns are split into kSlabs equal ranges
-//         and the matrix stored slab-major as kSlabs CSR pieces with absolute row pointers. Workgroup b works on
-//         slab b % 8, i.e. on the XCD the hardware sends it to, so each XCD gathers from a 1/8 slice of the
-//         vector that stays in its own L2; each slab writes partial row sums and a combine kernel adds the eight
-//         in a fixed order and applies the epilogue.
+//  slab:  for a gathered vector of at least kSlabMinCols entries (half an XCD's L2 in FP32), the columns are
+//         split into nslab equal ranges (slab_count: 8 or 16) and the matrix stored slab-major as nslab CSR pieces
+//         with absolute row pointers. Workgroup b works on slab b % nslab, i.e. on XCD b % 8, so each XCD gathers
+//         from slices of the vector that stay in its own L2; each slab writes partial row sums and a combine
+//         kernel adds them in slab order and applies the epilogue.
 constexpr int kSlabs = kSlabsK;
 constexpr int kSlabMinCols = 1 << 19;  // 2 MB of FP32 gather target and up
 
@@ -461,17 +461,17 @@ template <class T>
 struct DevMat {
     int rows = 0, nblk = 0, maxblk = 0;
     bool slab = false;
-    int *ptr = nullptr, *idx = nullptr, *blk = nullptr;  // slab: ptr holds kSlabs * (rows + 1) entries
+    int *ptr = nullptr, *idx = nullptr, *blk = nullptr;  // slab: ptr holds nslab * (rows + 1) entries
     int *snblk = nullptr, *sboff = nullptr;              // slab: block count and block-list offset per slab
     T* val = nullptr;
-    T* part = nullptr;  // slab: kSlabs * rows partial sums
+    T* part = nullptr;  // slab: nslab * rows partial sums
     int nch = 0, nlr = 0;                                // plain: chunks of long rows, long rows
     int *chunk = nullptr, *lrow = nullptr;
     T* lpart = nullptr;                                  // plain: one partial sum per chunk
     bool sorted = false;                                 // idx holds packed (position, column) words
     int cbits = 0, sb = 0;
     int* sstart = nullptr;                               // sorted slab: first column of each slab
-    int nslab = 0;                                       // slab: number of column slabs (a divisor of 8)
+    int nslab = 0;                                       // slab: number of column slabs (2, 4, 8 or 16)
 };
 
 struct HostLayout {
@@ -486,17 +486,18 @@ struct HostLayout {
     int nslab = 0;
 };
 
-// Column slabs of the slab layout: one per XCD. Measured at config 1 (FP32, sorted slab blocks with batched
-// gathers): A x gathering 2M entries 166 us with 8 slabs against 207 with 4 and 254 with 2; A^T z gathering 1M
-// entries 154 us with 8 against 166 with 4 and 206 with 2 (more slabs cost partial sums and row pointers, fewer
-// slabs give each XCD a larger slice to gather from). Workgroup b works on slab b mod nslab, so with fewer than 8
-// slabs XCDs x and x + nslab share one. TPDHG_SLABS=2|4|8 forces a count.
-int slab_count(int) {
+// Column slabs of the slab layout: the fewest of 8 and 16 that keep a slab's slice of the gathered vector within
+// 2^17 entries. Measured at config 1 (FP32, sorted slab blocks with batched gathers): A x gathering 2M entries
+// 155 us with 16 slabs, 166 with 8, 207 with 4; A^T z gathering 1M entries 154 us with 8, 186 with 16, 166 with
+// 4 (narrower slices pack a block's gathers into fewer cache lines; more slabs cost partial sums and row
+// pointers). Workgroup b works on slab b mod nslab, i.e. on XCD b mod 8: with 16 slabs each XCD serves two
+// slices, with fewer than 8 XCDs x and x + nslab share one. TPDHG_SLABS=2|4|8|16 forces a count.
+int slab_count(int cols) {
     if (const char* e = std::getenv("TPDHG_SLABS")) {
         const int k = std::atoi(e);
-        if (k == 2 || k == 4 || k == 8) return k;
+        if (k == 2 || k == 4 || k == 8 || k == 16) return k;
     }
-    return kSlabs;
+    return int64_t(cols) > int64_t(kSlabs) << 17 ? 2 * kSlabs : kSlabs;
 }
 
 // Largest sorted blocks. Plain layout (256 threads): 8,192 (32 KB of FP32 products in LDS; measured at config 1
