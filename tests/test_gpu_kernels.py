@@ -1,6 +1,7 @@
 """HIP kernels against the CPU oracle, through the C ABI (libtaral_pdhg.so).
 
-SpMV (both layouts: column-sorted blocks and row blocks, plain and XCD-slab, long rows): FP64 within
+SpMV (both layouts: column-sorted blocks and row blocks, plain and XCD-slab with 8 and 16 slabs, long rows, the
+largest blocks and workgroups): FP64 within
 summation-order error of the FP64 product; FP32 within (k + 2) 2^-24 sum |a_k x_k| of the product of the
 FP32-rounded inputs (k = row length). Backend operations (power iteration, start, one
 64-step iteration block, fused current+average evaluation, restart, download) against the oracle's CPU
@@ -57,6 +58,38 @@ def test_spmv_parity(case, bits, layout, monkeypatch):
     ptr, idx, val = _csr_from_rows(lengths, cols, rng)
     v = rng.standard_normal(cols)
     print(f"spmv case={case} bits={bits} layout={layout} rows={ptr.size - 1} nnz={ptr[-1]}", flush=True)
+    y, _, _ = ffi.spmv_bench(ptr, idx, val, v, precision_bits=bits, reps=2)
+    if bits == 64:
+        ref, mag = _spmv_ref(ptr, idx, val, v)
+        tol = 4 * np.diff(ptr) * 2.0 ** -53 * mag + 1e-300
+    else:
+        v32, val32 = v.astype(np.float32).astype(np.float64), val.astype(np.float32).astype(np.float64)
+        ref, mag = _spmv_ref(ptr, idx, val32, v32)
+        tol = (np.diff(ptr) + 2) * 2.0 ** -24 * mag + 1e-30
+    assert np.all(np.abs(y - ref) <= tol)
+
+
+LARGE_CASES = {  # (rows, cols, mean row length): sizes that reach the largest blocks and workgroups
+    # 16 column slabs (gathered vector > 2^20 entries), 16,384-nonzero blocks (FP32) on 1,024 threads, 8,192 (FP64)
+    "slab16_big_blocks": (900_000, 1_200_000, 20),
+    # plain sorted layout (< 2^19 columns) with 8,192-nonzero blocks on 512 threads
+    "plain_big_blocks": (100_000, 400_000, 86),
+}
+
+
+@pytest.mark.parametrize("layout", ["sorted", "unsorted"])
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("case", sorted(LARGE_CASES))
+def test_spmv_parity_large(case, bits, layout, monkeypatch):
+    monkeypatch.setenv("TPDHG_LAYOUT", layout)
+    rows, cols, mean = LARGE_CASES[case]
+    rng = np.random.default_rng(12)
+    lengths = rng.poisson(mean, rows)
+    ptr = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
+    idx = rng.integers(0, cols, int(ptr[-1])).astype(np.int32)  # repeated columns in a row are summed
+    val = rng.standard_normal(int(ptr[-1]))
+    v = rng.standard_normal(cols)
+    print(f"spmv large case={case} bits={bits} layout={layout} rows={rows} cols={cols} nnz={ptr[-1]}", flush=True)
     y, _, _ = ffi.spmv_bench(ptr, idx, val, v, precision_bits=bits, reps=2)
     if bits == 64:
         ref, mag = _spmv_ref(ptr, idx, val, v)
