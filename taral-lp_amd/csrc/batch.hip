@@ -613,7 +613,9 @@ private:
         mtot_ = int(mt), ntot_ = int(nt);
         auto add = [](Pool& pool, const Csr& a) {
             const int64_t at = int64_t(pool.val.size());
-            if (at + a.nnz() > INT32_MAX) throw std::runtime_error("batch matrices exceed 2^31 stored nonzeros");
+            if (at + a.nnz() > kMaxNnz)
+                throw std::length_error("the batch's distinct matrices hold more than " + std::to_string(kMaxNnz) +
+                                        " nonzeros (32-bit nonzero indices)");
             pool.pbase.push_back(int32_t(pool.ptr.size()));
             pool.rows.push_back(a.rows);
             for (int32_t p : a.ptr) pool.ptr.push_back(int32_t(at + p));

@@ -13,6 +13,8 @@
 #include <string>
 #include <vector>
 
+#include "prep.hpp"
+
 namespace tpdhg {
 namespace {
 
@@ -561,6 +563,9 @@ void sort_blocks(const std::vector<int32_t>& blk, const int32_t* ptr, const int3
                                          int value_bytes = 4) {
     HostLayout L;
     const int64_t nnz = ptr[rows];
+    if (nnz > kMaxNnz)
+        throw std::length_error("a matrix of " + std::to_string(nnz) + " nonzeros exceeds the " +
+                                std::to_string(kMaxNnz) + " the kernels index with 32-bit integers");
     L.slab = cols >= kSlabMinCols && rows > 0;
     const LayoutMode mode = layout_mode();
     const bool want_sorted = mode == LayoutMode::sorted || (mode == LayoutMode::automatic && nnz >= kSortMinNnz);

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
+#include <string>
 
 namespace tpdhg {
 
@@ -108,6 +110,9 @@ Scaled build_scaled_csc(int m, int n, const int64_t* colptr, const int32_t* rowi
     T.ptr.assign(size_t(n) + 1, 0);
     int64_t keep = 0;
     for (int64_t k = 0; k < colptr[n]; ++k) keep += val[k] != 0;
+    if (keep > kMaxNnz)
+        throw std::length_error("the constraint matrix has " + std::to_string(keep) + " nonzeros; at most " +
+                                std::to_string(kMaxNnz) + " are supported (32-bit nonzero indices)");
     T.idx.reserve(size_t(keep)), T.val.reserve(size_t(keep));
     for (int j = 0; j < n; ++j) {
         for (int64_t k = colptr[j]; k < colptr[j + 1]; ++k)

@@ -8,6 +8,10 @@
 
 namespace tpdhg {
 
+// Most nonzeros one matrix (or one batch pool) may hold: nonzeros are indexed with 32-bit integers on the host and
+// the device, and the device's staging loops form indices up to 2^16 past a block's last nonzero before clamping.
+constexpr int64_t kMaxNnz = int64_t(INT32_MAX) - (int64_t(1) << 16);
+
 struct Csr {  // rows x cols, column indices ascending within each row
     int rows = 0;
     std::vector<int32_t> ptr, idx;
