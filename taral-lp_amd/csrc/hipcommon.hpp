@@ -412,9 +412,13 @@ std::vector<int32_t> row_blocks(int rows, const int32_t* ptr) {
     return b;
 }
 
-// Row blocks of the sorted layout: maximal runs of rows with at most sb nonzeros, no row longer than long_above
-// inside a run; a row longer than long_above forms a block of its own (long_above <= sb).
+// Row blocks of the sorted layout: maximal runs of rows with at most sb nonzeros and at most 8 rows per thread of
+// the block's workgroup (stage_wg(sb)), no row longer than long_above inside a run; a row longer than long_above
+// forms a block of its own (long_above <= sb). The row cap bounds the row-sum phase where many rows are empty (a
+// block-structured matrix cut into column slabs leaves most rows of a slab empty; without it one workgroup once
+// wrote 280,000 empty rows' sums and a product took 316 instead of 23 us).
 std::vector<int32_t> sorted_blocks(int rows, const int32_t* ptr, int sb, int long_above = kCap) {
+    const int max_rows = 8 * stage_wg(sb);
     std::vector<int32_t> b;
     b.push_back(0);
     int r0 = 0;
@@ -423,7 +427,9 @@ std::vector<int32_t> sorted_blocks(int rows, const int32_t* ptr, int sb, int lon
         if (int64_t(ptr[r0 + 1]) - ptr[r0] > long_above) {
             r = r0 + 1;
         } else {
-            while (r < rows && int64_t(ptr[r + 1]) - ptr[r] <= long_above && int64_t(ptr[r + 1]) - ptr[r0] <= sb) ++r;
+            while (r < rows && r - r0 < max_rows && int64_t(ptr[r + 1]) - ptr[r] <= long_above &&
+                   int64_t(ptr[r + 1]) - ptr[r0] <= sb)
+                ++r;
             if (r == r0) r = r0 + 1;
         }
         b.push_back(r);
