@@ -101,6 +101,41 @@ def test_spmv_parity_large(case, bits, layout, monkeypatch):
     assert np.all(np.abs(y - ref) <= tol)
 
 
+@pytest.mark.parametrize("layout", ["sorted", "unsorted"])
+@pytest.mark.parametrize("bits", [32, 64])
+def test_spmv_parity_block_structured(bits, layout, monkeypatch):
+    """A block-diagonal matrix with more than 2^19 columns plus a few rows spanning all blocks: most (row, slab)
+    pairs are empty, so the layout is one piece (row blocks) instead of column slabs; its transpose likewise."""
+    monkeypatch.setenv("TPDHG_LAYOUT", layout)
+    rng = np.random.default_rng(13)
+    blocks, rows_b, cols_b = 64, 1200, 10000
+    rows, cols = blocks * rows_b + 8, blocks * cols_b
+    lengths = np.concatenate([rng.integers(1, 25, blocks * rows_b), np.full(8, 3000)])
+    ptr = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
+    blk = np.concatenate([np.repeat(np.arange(blocks * rows_b) // rows_b, lengths[:-8]), np.full(8 * 3000, -1)])
+    idx = np.where(blk >= 0, blk * cols_b + rng.integers(0, cols_b, blk.size), rng.integers(0, cols, blk.size))
+    idx = idx.astype(np.int32)
+    val = rng.standard_normal(int(ptr[-1]))
+    from scipy.sparse import csr_matrix
+    T = csr_matrix((val, idx, ptr), shape=(rows, cols)).T.tocsr()  # repeated entries kept apart
+    T.sort_indices()
+    mats = {"A": (ptr, idx, val, cols),
+            "AT": (T.indptr.astype(np.int32), T.indices.astype(np.int32), T.data.astype(np.float64), rows)}
+    for name, (p_, i_, v_, n_) in mats.items():
+        x = rng.standard_normal(n_)
+        print(f"spmv block-structured {name} bits={bits} layout={layout} rows={p_.size - 1} cols={n_} nnz={p_[-1]}",
+              flush=True)
+        y, _, _ = ffi.spmv_bench(p_, i_, v_, x, precision_bits=bits, reps=2)
+        if bits == 64:
+            ref, mag = _spmv_ref(p_, i_, v_, x)
+            tol = 4 * np.diff(p_) * 2.0 ** -53 * mag + 1e-300
+        else:
+            x32, v32 = x.astype(np.float32).astype(np.float64), v_.astype(np.float32).astype(np.float64)
+            ref, mag = _spmv_ref(p_, i_, v32, x32)
+            tol = (np.diff(p_) + 2) * 2.0 ** -24 * mag + 1e-30
+        assert np.all(np.abs(y - ref) <= tol)
+
+
 @pytest.fixture(scope="module")
 def packing_problem():
     md, _ = gen.packing_mps(10000)
