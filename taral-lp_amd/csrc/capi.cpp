@@ -83,6 +83,35 @@ tpdhg::Options to_options(const tpdhg_options* o) {
     r.omega_rule = o->omega_rule;
     return r;
 }
+
+// A CSR the kernels can index without leaving its arrays: ptr[0] = 0, nondecreasing row pointers ending at the
+// nonzero count (`nnz` when it is >= 0), every column index in [0, cols), no null array behind a nonzero. Returns
+// the first violation, or nullptr. The kernels trust these arrays, so every entry point that takes a caller's CSR
+// checks them on the host first.
+const char* csr_error(int32_t rows, int32_t cols, const int32_t* ptr, const int32_t* idx, const double* val,
+                      int64_t nnz) {
+    if (rows < 0 || cols < 0) return "negative dimensions";
+    if (!ptr) return "null row pointers";
+    if (ptr[0] != 0) return "row pointers must start at 0";
+    for (int32_t r = 0; r < rows; ++r)
+        if (ptr[r + 1] < ptr[r]) return "row pointers must not decrease";
+    if (nnz >= 0 && ptr[rows] != nnz) return "row pointers do not end at the nonzero count";
+    if (int64_t(ptr[rows]) > tpdhg::kMaxNnz) return "too many nonzeros for 32-bit nonzero indices";
+    if (ptr[rows] > 0 && (!idx || !val)) return "null column indices or values";
+    for (int32_t k = 0; k < ptr[rows]; ++k)
+        if (idx[k] < 0 || idx[k] >= cols) return "column index out of range";
+    return nullptr;
+}
+
+// The scaled problem arrays of tpdhg_dev_create / tpdhg_fast_create: both CSRs and every vector.
+const char* problem_error(const tpdhg_scaled_arrays* p) {
+    if (p->m < 0 || p->n < 0 || p->nnz < 0) return "negative dimensions";
+    if (const char* e = csr_error(p->m, p->n, p->a_ptr, p->a_idx, p->a_val, p->nnz)) return e;
+    if (const char* e = csr_error(p->n, p->m, p->at_ptr, p->at_idx, p->at_val, p->nnz)) return e;
+    if (p->n > 0 && (!p->c || !p->l || !p->u || !p->Dc)) return "null column vector";
+    if (p->m > 0 && (!p->rl || !p->ru || !p->Dr)) return "null row vector";
+    return nullptr;
+}
 }  // namespace
 
 extern "C" {
@@ -216,7 +245,8 @@ void tpdhg_scaled_free(tpdhg_scaled* s) { delete s; }
 
 int tpdhg_dev_create(const tpdhg_scaled_arrays* p, int32_t precision_bits, int32_t eval_freq, int32_t use_graph,
                      tpdhg_dev** out) {
-    if (!p || !out || p->m < 0 || p->n < 0 || !p->a_ptr || !p->at_ptr) return fail(TPDHG_ERR_ARG, "bad problem");
+    if (!p || !out) return fail(TPDHG_ERR_ARG, "bad problem");
+    if (const char* e = problem_error(p)) return fail(TPDHG_ERR_ARG, std::string("bad problem: ") + e);
     return guarded([&]() -> int {
         tpdhg::DeviceOptions o;
         o.precision_bits = precision_bits, o.eval_freq = eval_freq, o.use_graph = use_graph != 0;
@@ -270,7 +300,8 @@ void tpdhg_dev_destroy(tpdhg_dev* dev) { delete dev; }
 
 int tpdhg_fast_create(const tpdhg_scaled_arrays* p, int32_t precision_bits, int32_t eval_freq, int32_t use_graph,
                       double reflection, tpdhg_fast** out) {
-    if (!p || !out || p->m < 0 || p->n < 0 || !p->a_ptr || !p->at_ptr) return fail(TPDHG_ERR_ARG, "bad problem");
+    if (!p || !out) return fail(TPDHG_ERR_ARG, "bad problem");
+    if (const char* e = problem_error(p)) return fail(TPDHG_ERR_ARG, std::string("bad problem: ") + e);
     return guarded([&]() -> int {
         tpdhg::Scaled S;  // owned copy of the caller's arrays
         S.m = p->m, S.n = p->n;
@@ -558,6 +589,7 @@ int tpdhg_spmv_bench(int32_t precision_bits, int32_t rows, int32_t cols, const i
                      const double* val, const double* v, double* y, int32_t reps, double* mean_ms, double* median_ms) {
     if (rows < 0 || cols < 0 || !ptr || !v || !y || !mean_ms || !median_ms || reps < 1)
         return fail(TPDHG_ERR_ARG, "bad arguments");
+    if (const char* e = csr_error(rows, cols, ptr, idx, val, -1)) return fail(TPDHG_ERR_ARG, std::string("bad CSR: ") + e);
     return guarded([&]() -> int {
         tpdhg::spmv_bench(precision_bits, rows, cols, ptr, idx, val, v, y, reps, mean_ms, median_ms);
         return TPDHG_OK;

@@ -152,3 +152,59 @@ def test_cli_batch_list_gates_before_the_device(tmp_path):
         lst.write_text(os.path.join(MPS, "gate_continuous_lp.mps") + "\n")
         r = subprocess.run([ffi.CLI_PATH, "--batch", str(lst)], capture_output=True, text=True)
         assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
+def _arrays(m, n, a, at):
+    """A tpdhg_scaled_arrays over numpy arrays (kept alive in the returned tuple)."""
+    import ctypes as C
+    import numpy as np
+    keep = [np.ascontiguousarray(x) for x in (*a, *at)] + [np.ones(max(n, 1)), np.ones(max(m, 1))]
+    d = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+    i = lambda x: x.ctypes.data_as(C.POINTER(C.c_int32))
+    col, row = keep[6], keep[7]
+    s = ffi.ScaledArrays(m=m, n=n, nnz=int(a[0][-1]), a_ptr=i(keep[0]), a_idx=i(keep[1]), a_val=d(keep[2]),
+                         at_ptr=i(keep[3]), at_idx=i(keep[4]), at_val=d(keep[5]), c=d(col), l=d(col), u=d(col),
+                         rl=d(row), ru=d(row), Dr=d(row), Dc=d(col), bnorm=1.0, cnorm=1.0)
+    return s, keep
+
+
+def test_raw_csr_entry_points_validate_before_the_device():
+    """tpdhg_spmv_bench, tpdhg_dev_create and tpdhg_fast_create take a caller's CSR, which the kernels index
+    without bounds checks: a malformed one is refused on the host (TPDHG_ERR_ARG) before any device work; a valid
+    one gets as far as the device (TPDHG_ERR_NO_DEVICE here when no GPU is visible)."""
+    import ctypes as C
+    import numpy as np
+    ptr, idx, val = np.array([0, 2, 3], np.int32), np.array([0, 2, 1], np.int32), np.array([1.0, 2.0, 3.0])
+    v = np.ones(3)
+    bad = {
+        "row pointers must start at 0": (np.array([1, 2, 3], np.int32), idx, val),
+        "row pointers must not decrease": (np.array([0, 3, 2], np.int32), idx, val),
+        "column index out of range": (ptr, np.array([0, 3, 1], np.int32), val),
+    }
+    for msg, (p, i, x) in bad.items():
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.spmv_bench(p, i, x, v)
+        assert e.value.code == ffi.ERR_ARG and msg in str(e.value), msg
+    with pytest.raises(ffi.TpdhgError) as e:
+        ffi.spmv_bench(ptr, np.array([0, -1, 1], np.int32), val, v)
+    assert e.value.code == ffi.ERR_ARG and "column index out of range" in str(e.value)
+    # A: 2 x 3 with rows {0, 2} and {1}; A^T: 3 x 2
+    a = (ptr, idx, val)
+    at = (np.array([0, 1, 2, 3], np.int32), np.array([0, 1, 0], np.int32), np.array([1.0, 3.0, 2.0]))
+    at_bad = (at[0], np.array([0, 2, 0], np.int32), at[2])      # a row index of A^T beyond m = 2
+    at_short = (np.array([0, 1, 2, 2], np.int32), at[1], at[2])  # ends at 2 nonzeros, A has 3
+    L = ffi.lib()
+    for arr, msg in ((at_bad, "column index out of range"), (at_short, "nonzero count")):
+        s, keep = _arrays(2, 3, a, arr)
+        h = C.c_void_p()
+        assert L.tpdhg_dev_create(C.byref(s), 64, 64, 1, C.byref(h)) == ffi.ERR_ARG
+        assert msg in L.tpdhg_last_error().decode()
+        assert L.tpdhg_fast_create(C.byref(s), 32, 64, 1, 1.9, C.byref(h)) == ffi.ERR_ARG
+        assert msg in L.tpdhg_last_error().decode()
+    if ffi.device_count() == 0:
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.spmv_bench(ptr, idx, val, v)
+        assert e.value.code == ffi.ERR_NO_DEVICE
+        s, keep = _arrays(2, 3, a, at)
+        h = C.c_void_p()
+        assert L.tpdhg_dev_create(C.byref(s), 64, 64, 1, C.byref(h)) == ffi.ERR_NO_DEVICE
