@@ -55,10 +55,8 @@ struct EpiStore {  // y = M v
 // index words and values, then its U gathered entries, all at indices clamped into [k0, k1) so that no load sits
 // behind a bounds test, and only then hands the in-range products to use(k, word, product) in the order of k.
 // With a bounds test around each load the compiler waits for every gather before issuing the next one (one load
-// chain in flight per wave); this way a wave keeps 2U and then U loads in flight. NT marks the streamed index and
-// value loads nontemporal, for matrices far larger than the caches, so that they do not displace the gathered
-// vector's lines from L2 (config 1 slabs: A x 156 -> 151 us, A^T z 154 -> 152 us, same results).
-template <int WG, int U, bool NT = false, class T, class W, class Col, class Use>
+// chain in flight per wave); this way a wave keeps 2U and then U loads in flight.
+template <int WG, int U, class T, class W, class Col, class Use>
 __device__ __forceinline__ void gather_rounds(int k0, int k1, const W* __restrict__ w, const T* __restrict__ val,
                                               const T* __restrict__ v, Col col, Use use) {
     if (k1 <= k0) return;
@@ -69,13 +67,8 @@ __device__ __forceinline__ void gather_rounds(int k0, int k1, const W* __restric
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = min(base + u * WG + tid, k1 - 1);
-            if constexpr (NT) {
-                p[u] = __builtin_nontemporal_load(w + k);
-                a[u] = __builtin_nontemporal_load(val + k);
-            } else {
-                p[u] = w[k];
-                a[u] = val[k];
-            }
+            p[u] = w[k];
+            a[u] = val[k];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) g[u] = v[col(p[u])];
@@ -269,8 +262,7 @@ __global__ __launch_bounds__(WG) void k_sorted_slab(int rows, int nslab, const i
         if (tid == 0) y[r0] = tree_sum<WG / 64>(red);
         return;
     }
-    gather_rounds<WG, 8, true>(k0, k1, pk, val, vq, ColPacked{mask},
-                               [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
+    gather_rounds<WG, 8>(k0, k1, pk, val, vq, ColPacked{mask}, [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
     __syncthreads();
     block_row_sums<T, WG>(ptr, r0, r1, k0, sh, [&](int r, T s) { y[r] = s; });
 }
