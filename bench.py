@@ -92,7 +92,7 @@ def mode_options(mode):
     return dict(MODES[mode])
 
 
-CPU_THREADS = 16  # the GPU box's CPU share per GPU
+CPU_THREADS = 16  # host threads for the CPU baseline: 16 cores per GPU
 
 
 def cpu_threads():
@@ -101,7 +101,7 @@ def cpu_threads():
 
 def cpu_baseline(md, iterations, sample_iters):
     """Oracle FP64 on the host's cores (the reference CPU backend's row partition and barriers, one thread per
-    core of the box's share): time PDHG iterations of the same problem (at least `sample_iters`, doubled until the
+    core, at most CPU_THREADS): time PDHG iterations of the same problem (at least `sample_iters`, doubled until the
     sample takes two seconds of wall time), scale to one solve of `iterations` iterations."""
     from oracle import oracle
     nt = cpu_threads()
@@ -464,8 +464,8 @@ def main():
         print(json.dumps({"copy_bytes": nbytes, "ms": dt * 1e3, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU sharding is a design only "
-              "(see the sharding section of the design notes)", file=sys.stderr)
+        print("bench.py: this solver runs on one GPU; sharding the rows of A and A^T over several GPUs (with "
+              "all-gathers of the iterate vectors) is not built", file=sys.stderr)
         sys.exit(2)
     from taral_pdhg import ffi, gen
     kind, desc, make = WORKLOADS[a.workload]
