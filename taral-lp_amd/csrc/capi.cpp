@@ -264,19 +264,27 @@ int tpdhg_dev_create(const tpdhg_scaled_arrays* p, int32_t precision_bits, int32
         return guarded([&]() -> int { body; return TPDHG_OK; });               \
     } while (0)
 
-int tpdhg_dev_power_norm(tpdhg_dev* dev, int32_t iters, double* norm) { DEV_CALL(dev, *norm = dev->d->power_norm(iters)); }
+int tpdhg_dev_power_norm(tpdhg_dev* dev, int32_t iters, double* norm) {
+    if (!norm || iters < 1) return fail(TPDHG_ERR_ARG, "null norm or iters < 1");
+    DEV_CALL(dev, *norm = dev->d->power_norm(iters));
+}
 int tpdhg_dev_set_steps(tpdhg_dev* dev, double tau, double sigma) { DEV_CALL(dev, dev->d->set_steps(tau, sigma)); }
 int tpdhg_dev_start(tpdhg_dev* dev, const double* x0, const double* z0) { DEV_CALL(dev, dev->d->start(x0, z0)); }
 int tpdhg_dev_iterate(tpdhg_dev* dev, int32_t steps) { DEV_CALL(dev, dev->d->iterate(steps)); }
 int tpdhg_dev_eval(tpdhg_dev* dev, int32_t avg, double f, double out[10]) {
+    if (!out) return fail(TPDHG_ERR_ARG, "null output");
     DEV_CALL(dev, if (avg) dev->d->eval2(f, nullptr, out); else dev->d->eval2(f, out, nullptr));
 }
-int tpdhg_dev_eval2(tpdhg_dev* dev, double f, double cur[10], double avg[10]) { DEV_CALL(dev, dev->d->eval2(f, cur, avg)); }
+int tpdhg_dev_eval2(tpdhg_dev* dev, double f, double cur[10], double avg[10]) {
+    if (!cur && !avg) return fail(TPDHG_ERR_ARG, "null outputs");
+    DEV_CALL(dev, dev->d->eval2(f, cur, avg));
+}
 int tpdhg_dev_restart(tpdhg_dev* dev, int32_t avg, double f) { DEV_CALL(dev, dev->d->restart(avg != 0, f)); }
 int tpdhg_dev_download(tpdhg_dev* dev, int32_t avg, double f, double* x, double* z) {
     DEV_CALL(dev, dev->d->download(avg != 0, f, x, z));
 }
 int tpdhg_dev_spmv(tpdhg_dev* dev, int32_t transpose, const double* v, double* y) {
+    if (!v || !y) return fail(TPDHG_ERR_ARG, "null vector");
     DEV_CALL(dev, dev->d->spmv(transpose ? 1 : 0, v, y));
 }
 int tpdhg_dev_info(tpdhg_dev* dev, int64_t* mem_used, char* name_buf, int32_t name_len) {
@@ -329,13 +337,19 @@ int tpdhg_fast_create(const tpdhg_scaled_arrays* p, int32_t precision_bits, int3
         return guarded([&]() -> int { body; return TPDHG_OK; });            \
     } while (0)
 
-int tpdhg_fast_power_norm(tpdhg_fast* f, int32_t iters, double* norm) { FAST_CALL(f, *norm = f->d->power_norm(iters)); }
+int tpdhg_fast_power_norm(tpdhg_fast* f, int32_t iters, double* norm) {
+    if (!norm || iters < 1) return fail(TPDHG_ERR_ARG, "null norm or iters < 1");
+    FAST_CALL(f, *norm = f->d->power_norm(iters));
+}
 int tpdhg_fast_set_base(tpdhg_fast* f, const double* x_b, const double* z_b) { FAST_CALL(f, f->d->set_base(x_b, z_b)); }
 int tpdhg_fast_begin_round(tpdhg_fast* f) { FAST_CALL(f, f->d->begin_round()); }
 int tpdhg_fast_set_steps(tpdhg_fast* f, double tau, double sigma) { FAST_CALL(f, f->d->set_steps(tau, sigma)); }
 int tpdhg_fast_restart(tpdhg_fast* f) { FAST_CALL(f, f->d->restart()); }
 int tpdhg_fast_iterate(tpdhg_fast* f, int32_t steps) { FAST_CALL(f, f->d->iterate(steps)); }
-int tpdhg_fast_eval(tpdhg_fast* f, double out[10]) { FAST_CALL(f, f->d->eval(out)); }
+int tpdhg_fast_eval(tpdhg_fast* f, double out[10]) {
+    if (!out) return fail(TPDHG_ERR_ARG, "null output");
+    FAST_CALL(f, f->d->eval(out));
+}
 int tpdhg_fast_end_round(tpdhg_fast* f) { FAST_CALL(f, f->d->end_round()); }
 int tpdhg_fast_download(tpdhg_fast* f, double* x, double* z) { FAST_CALL(f, f->d->download(x, z)); }
 void tpdhg_fast_destroy(tpdhg_fast* f) { delete f; }
@@ -571,16 +585,17 @@ int tpdhg_mlp_get_params(tpdhg_mlp* net, float* p) {
     MLP_CALL(net, net->net->get_params(p));
 }
 int tpdhg_mlp_forward(tpdhg_mlp* net, const float* x, int32_t batch, float* y) {
-    if (!x || !y) return fail(TPDHG_ERR_ARG, "null argument");
+    if (!x || !y || batch < 1) return fail(TPDHG_ERR_ARG, "null argument or batch < 1");
     MLP_CALL(net, net->net->forward(x, batch, y));
 }
 int tpdhg_mlp_loss_grad(tpdhg_mlp* net, const float* x, const float* t, int32_t batch, double* loss, float* grad) {
-    if (!x || !t) return fail(TPDHG_ERR_ARG, "null argument");
+    if (!x || !t || batch < 1) return fail(TPDHG_ERR_ARG, "null argument or batch < 1");
     MLP_CALL(net, { const double l = net->net->loss_grad(x, t, batch, grad); if (loss) *loss = l; });
 }
 int tpdhg_mlp_train(tpdhg_mlp* net, const float* x, const float* t, int32_t n, int32_t batch, int32_t epochs, float lr,
                     uint64_t seed, double* loss) {
-    if (!x || !t) return fail(TPDHG_ERR_ARG, "null argument");
+    if (!x || !t || n < 1 || batch < 1 || epochs < 0 || !(lr > 0))
+        return fail(TPDHG_ERR_ARG, "null argument, or n, batch, epochs or lr out of range");
     MLP_CALL(net, { const double l = net->net->train(x, t, n, batch, epochs, lr, seed); if (loss) *loss = l; });
 }
 void tpdhg_mlp_destroy(tpdhg_mlp* net) { delete net; }

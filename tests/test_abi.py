@@ -208,3 +208,23 @@ def test_raw_csr_entry_points_validate_before_the_device():
         s, keep = _arrays(2, 3, a, at)
         h = C.c_void_p()
         assert L.tpdhg_dev_create(C.byref(s), 64, 64, 1, C.byref(h)) == ffi.ERR_NO_DEVICE
+
+
+def test_handle_calls_refuse_null_outputs_and_bad_sizes():
+    """Entry points that write through caller pointers or take sizes refuse null outputs and non-positive sizes
+    with TPDHG_ERR_ARG before touching a handle or a device (no crash on a caller's mistake)."""
+    import ctypes as C
+    L = ffi.lib()
+    nd = C.POINTER(C.c_double)()
+    nf = C.POINTER(C.c_float)()
+    assert L.tpdhg_dev_power_norm(None, 10, nd) == ffi.ERR_ARG
+    assert L.tpdhg_dev_power_norm(None, 0, C.byref(C.c_double())) == ffi.ERR_ARG
+    assert L.tpdhg_fast_power_norm(None, 10, nd) == ffi.ERR_ARG
+    assert L.tpdhg_dev_eval(None, 0, 1.0, nd) == ffi.ERR_ARG
+    assert L.tpdhg_dev_eval2(None, 1.0, nd, nd) == ffi.ERR_ARG
+    assert L.tpdhg_fast_eval(None, nd) == ffi.ERR_ARG
+    assert L.tpdhg_dev_spmv(None, 0, nd, nd) == ffi.ERR_ARG
+    x = (C.c_float * 32)()
+    assert L.tpdhg_mlp_forward(None, x, 0, x) == ffi.ERR_ARG
+    assert L.tpdhg_mlp_loss_grad(None, x, x, 0, None, nf) == ffi.ERR_ARG
+    assert L.tpdhg_mlp_train(None, x, x, 1, 1, 1, C.c_float(0.0), 0, None) == ffi.ERR_ARG
