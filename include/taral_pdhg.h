@@ -218,7 +218,7 @@ int tpdhg_session_set_vectors(tpdhg_session* s, const double* cost, const double
 int tpdhg_session_set_primal_weight(tpdhg_session* s, double primal_weight);
 void tpdhg_session_free(tpdhg_session* s);
 
-/* The JSON object gpu/pdhg.cu:881-897 writes (same keys and formats, plus this build's keys). */
+/* The JSON object gpu/pdhg.cu:881-897 writes (same keys and formats, plus the keys added here). */
 int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const tpdhg_result* result, char* buf,
                       int32_t len);
 

@@ -1,6 +1,6 @@
 """CPU oracle of the warm-start network (n_in -> h1 -> h2 -> n_out MLP, ReLU). TEST INFRASTRUCTURE ONLY.
 
-Imported by tests/ as the checker of the GPU network (taral-lp_amd/csrc/mlp.hip); never part of the product.
+Imported by tests/ as the oracle of the GPU network (taral-lp_amd/csrc/mlp.hip); never part of the product.
 The learned warm start has no counterpart in the reference, so there is no reference file to follow: the
 algorithm is textbook back-propagation of the mean squared error and Adam (Kingma & Ba, 2015, Algorithm 1).
 

@@ -1,6 +1,6 @@
 """ctypes wrapper of the CPU oracle (oracle/pdhg_oracle.c). TEST INFRASTRUCTURE ONLY.
 
-Imported by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, where it is the checker
+Imported by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, where it is the oracle
 (or the timed CPU baseline), never the thing measured on the GPU.
 """
 from __future__ import annotations

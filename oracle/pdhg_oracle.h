@@ -1,7 +1,7 @@
 /* CPU oracle for the restarted PDHG LP path: TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C restatement of the reference's host preparation and CPU backend, used by tests/, by
- * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker. No product code links or
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the oracle. No product code links or
  * loads it. Each function names the reference lines it follows (paths relative to the reference root).
  *
  * Pinning (which check fixes which function) is listed at each declaration; a function with no such check

@@ -1,6 +1,6 @@
 // Test probe: reads an MPS file with the REFERENCE reader (src/mps.cpp, compiled from /root/reference by
 // oracle/Makefile into oracle/_ref/, never committed) and prints the model in a canonical text form, so the
-// tests can compare this build's reader against it line for line. TEST INFRASTRUCTURE ONLY.
+// tests can compare the library's reader against it line for line. TEST INFRASTRUCTURE ONLY.
 #include <cmath>
 #include <cstdio>
 

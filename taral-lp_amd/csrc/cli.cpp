@@ -1,6 +1,6 @@
 // `pdhg` command line: a drop-in for the reference CLI (gpu/pdhg.cu:937-992). Same flags, stdout status line,
 // JSON keys, .sol/.dual files and exit codes (0 after any solve, 1 on a parse error, 2 on usage errors and
-// unsupported QP/MILP input, which is refused before any device work or output file). This build runs on the
+// unsupported QP/MILP input, which is refused before any device work or output file). The solver runs on the
 // GPU only: `--device cpu` is accepted by the parser (so scripts keep working) but ends with exit 2 and a
 // message, after the input gate, instead of running a CPU solver.
 //

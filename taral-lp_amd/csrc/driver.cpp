@@ -627,7 +627,7 @@ Report solve_lp(const LP& lp, const Options& o, Timings tm, Clock::time_point t_
 }
 
 std::string report_json(const Report& r, const LP& lp, const Options& o) {
-    // Keys and formats of gpu/pdhg.cu:883-895, then this build's additions.
+    // Keys and formats of gpu/pdhg.cu:883-895, then the keys added here.
     char buf[4096];
     const Outcome& q = r.out;
     const double iters_per_s = r.solve_s > 0 ? double(q.iterations) / r.solve_s : 0;

@@ -33,7 +33,7 @@ LP flatten(const Model& md);
 // Reason string when PDHG must refuse the model (quadratic or integer), nullptr otherwise.
 const char* input_error(const LP& lp);
 
-struct Options {  // defaults of gpu/pdhg.cu:744-750 plus this build's additions
+struct Options {  // defaults of gpu/pdhg.cu:744-750 plus the options added here
     double tol = 1e-4, time_limit = 3600;
     int64_t max_iter = 2000000000LL;
     int eval_freq = 64;

@@ -1,8 +1,8 @@
 """Writes tests/golden/cfg0_dual_simplex.json: the reference's FP64 dual-simplex optimum of the 2,000 x 5,000
 planted LP (10 nonzeros per column, seed 0).
 
-The reference CPU engine (src/*.cpp, built by `make -C oracle ref` into oracle/_ref/taral in this container
-only) is run as `taral MODEL.mps --method dual --json OUT` (src/main.cpp:425-712). Run from the repository
+The reference CPU engine (src/*.cpp, built by `make -C oracle ref` into oracle/_ref/taral where the reference
+tree is present) is run as `taral MODEL.mps --method dual --json OUT` (src/main.cpp:425-712). Run from the repository
 root: python tests/golden/make_dual_simplex_golden.py
 """
 import json

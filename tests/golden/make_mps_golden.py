@@ -5,7 +5,8 @@ oracle/_ref/ref_mps_dump) makes of the .mps file, printed in a canonical text fo
 oracle/ref_mps_dump.cpp). The cases are written here (edge cases of the MPS rules and the 13 input-gate
 models of gpu/pdhg_input_tests.py:57-71), plus data files copied from the reference's own tests:
 tests/repro/parser_*.mps (reader rejection fixtures) and examples/literature_lp/*.mps (small LPs with
-known optima). Run from the repository root in the build container: python tests/golden/make_mps_golden.py
+known optima). Run from the repository root where the reference tree is present:
+python tests/golden/make_mps_golden.py
 """
 import os
 import shutil

@@ -1,4 +1,4 @@
-"""MPS input contract: this build's reader against the reference reader's output on the same files.
+"""MPS input contract: the library's reader against the reference reader's output on the same files.
 
 Fixtures in tests/golden/mps: each <case>.dump is the reference reader's (src/mps.cpp) model for
 <case>.mps, written by tests/golden/make_mps_golden.py. Compared field by field: row/column names and
@@ -25,7 +25,7 @@ def _num(v):
 
 
 def dump_with_this_build(path):
-    """The reference dump format (oracle/ref_mps_dump.cpp) from this build's reader, minus the NAME line."""
+    """The reference dump format (oracle/ref_mps_dump.cpp) from the library's reader, minus the NAME line."""
     try:
         md = ffi.Model.from_mps(path)
     except ffi.TpdhgError as e:
@@ -52,7 +52,7 @@ def dump_with_this_build(path):
 def reference_dump(case):
     with open(os.path.join(MPS_DIR, case + ".dump")) as f:
         lines = f.read().splitlines()
-    # NAME is not part of this build's model API; quadratic terms are compared by count
+    # NAME is not part of the library's model API; quadratic terms are compared by count
     return [ln for ln in lines if not ln.startswith("name ") and not (ln.startswith("q ") and ln.count(" ") > 1)]
 
 

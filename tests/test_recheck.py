@@ -27,7 +27,7 @@ def test_planted_optimum_closes_the_gap(planted):
     assert r["dual_bound"] == pytest.approx(md.planted_objective, rel=1e-12, abs=1e-11)
     assert abs(r["dual_gap_rel"]) <= 1e-12 and r["dual_infeas_rel"] == 0
     # every column is boxed and every row an equality, so both signs of y give finite bounds: the search keeps
-    # the first (+) as the reference's checker does, and weak duality still holds for the poor multipliers -y*
+    # the first (+) as the reference's recheck (gpu/vector_check.py) does, and weak duality still holds for the poor multipliers -y*
     flipped = ffi.recheck(model, md.x_star, -md.y_star)
     assert flipped["dual_sign"] == 1 and flipped["dual_bound_finite"] == 1
     assert flipped["dual_bound"] <= md.planted_objective + 1e-9
