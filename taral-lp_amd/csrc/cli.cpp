@@ -207,7 +207,7 @@ int main(int argc, char** argv) {
     (void)threads;
     if (o.refine && !o.halpern) o.halpern = true;
     if (device == "cpu") {
-        std::fprintf(stderr, "this build solves on the GPU only; use --device gpu (there is no CPU backend)\n");
+        std::fprintf(stderr, "this solver runs on the GPU only; use --device gpu (there is no CPU backend)\n");
         return 2;
     }
     std::vector<double> x0, y0;
