@@ -1,6 +1,7 @@
 // Host driver of the restarted averaged PDHG (gpu/pdhg.cu:743-849) over the HIP device backend, and the
 // solve pipeline with its original-space report (gpu/pdhg.cu:851-911).
 #include "driver.hpp"
+#include "parallel.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -532,11 +533,13 @@ BatchSession::BatchSession(const std::vector<const LP*>& lps, const Options& o, 
         if (!lp) throw std::invalid_argument("null batch instance");
         lps_.push_back(*lp);
     }
-    P_.reserve(lps_.size());
-    for (const LP& lp : lps_)
-        P_.push_back(build_scaled_csc(lp.m, lp.n, lp.colptr.data(), lp.rowidx.data(), lp.val.data(), lp.cost.data(),
-                                      lp.col_lo.data(), lp.col_up.data(), lp.row_lo.data(), lp.row_up.data(),
-                                      lp.maximize ? -1.0 : 1.0));
+    P_.resize(lps_.size());
+    parallel_for(int64_t(lps_.size()), 1, [&](int64_t i) {  // instances scale independently
+        const LP& lp = lps_[size_t(i)];
+        P_[size_t(i)] = build_scaled_csc(lp.m, lp.n, lp.colptr.data(), lp.rowidx.data(), lp.val.data(),
+                                         lp.cost.data(), lp.col_lo.data(), lp.col_up.data(), lp.row_lo.data(),
+                                         lp.row_up.data(), lp.maximize ? -1.0 : 1.0);
+    });
     if (device_count() == 0)
         throw std::runtime_error("no HIP device visible: the PDHG solver runs on an AMD GPU only (there is no CPU path)");
     BatchOptions bo;

@@ -16,6 +16,7 @@
 #include <thread>
 #include <vector>
 
+#include "parallel.hpp"
 #include "prep.hpp"
 
 namespace tpdhg {
@@ -546,35 +547,6 @@ LayoutMode layout_mode() {
 
 // Sorts the entries of every block of a row range by column, writing packed words and the permutation (entries
 // [base + k0, base + k1) of idx_in map to the same range of pk / perm; positions are relative to the block).
-// Runs f(i) for i in [0, n) on up to 16 host threads, `grain` consecutive indices at a time; f must write disjoint
-// data for different i. An exception thrown by f is rethrown here after every thread has stopped.
-template <class F>
-void parallel_for(int64_t n, int64_t grain, F&& f) {
-    const int64_t chunks = (n + grain - 1) / grain;
-    const int64_t hw = std::max<int64_t>(1, std::min<int64_t>(16, int64_t(std::thread::hardware_concurrency())));
-    const int nt = int(std::min(chunks, hw));
-    if (nt <= 1) {
-        for (int64_t i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::atomic<int64_t> next{0};
-    std::atomic<bool> failed{false};
-    std::exception_ptr err;
-    auto work = [&] {
-        try {
-            for (int64_t c = next.fetch_add(1); c < chunks && !failed; c = next.fetch_add(1))
-                for (int64_t i = c * grain, e = std::min(n, (c + 1) * grain); i < e; ++i) f(i);
-        } catch (...) {
-            if (!failed.exchange(true)) err = std::current_exception();
-        }
-    };
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; ++t) th.emplace_back(work);
-    work();
-    for (std::thread& t : th) t.join();
-    if (err) std::rethrow_exception(err);
-}
-
 void sort_blocks(const std::vector<int32_t>& blk, const int32_t* ptr, const int32_t* cols_local, int64_t perm_base_in,
                  const int64_t* perm_in, int cbits, std::vector<int32_t>& pk, std::vector<int64_t>& perm,
                  int long_above = kCap) {

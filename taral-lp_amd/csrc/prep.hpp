@@ -19,6 +19,12 @@ struct Csr {  // rows x cols, column indices ascending within each row
     int64_t nnz() const { return int64_t(val.size()); }
 };
 
+// Column order of a's entries: column j holds positions [cptr[j], cptr[j+1]), position q is the CSR entry
+// cperm[q] of row (*crow)[q], rows ascending within a column (a counting sort, done in row chunks on the host's
+// cores with the sequential sort's result).
+void column_order(const Csr& a, int cols, std::vector<int32_t>& cptr, std::vector<int32_t>& cperm,
+                  std::vector<int32_t>* crow);
+
 // Counting-sort transpose; `cols` is the column count of `a` (gpu/pdhg.cu:55-70).
 Csr transpose(const Csr& a, int cols);
 
