@@ -739,12 +739,12 @@ private:
                         cap_candidate_);
         }
         std::vector<double> pv(nnz);
-        for (size_t k = 0; k < nnz; ++k) pv[k] = pool.val[size_t(perm[k])];
+        parallel_for(int64_t(nnz), 1 << 16, [&](int64_t k) { pv[size_t(k)] = pool.val[size_t(perm[size_t(k)])]; });
         Side& sd = a_side ? A_ : AT_;
         Side64& s64 = a_side ? A64_ : AT64_;
         if (nnz) HIP_OK(hipMemcpyAsync(sd.idx, pk.data(), nnz * sizeof(int), hipMemcpyHostToDevice, st_));
         std::vector<T> tv(nnz);
-        for (size_t k = 0; k < nnz; ++k) tv[k] = T(pv[k]);
+        parallel_for(int64_t(nnz), 1 << 16, [&](int64_t k) { tv[size_t(k)] = T(pv[size_t(k)]); });
         if (nnz) HIP_OK(hipMemcpyAsync(sd.val, tv.data(), nnz * sizeof(T), hipMemcpyHostToDevice, st_));
         if constexpr (sizeof(T) != sizeof(double))
             if (nnz) HIP_OK(hipMemcpyAsync(s64.val, pv.data(), nnz * sizeof(double), hipMemcpyHostToDevice, st_));

@@ -674,9 +674,9 @@ T* upload_values(Mem& mem, hipStream_t st, const HostLayout& L, const double* va
     if (nnz) {
         std::vector<T> h(static_cast<size_t>(nnz));
         if (!L.perm.empty())
-            for (int64_t k = 0; k < nnz; ++k) h[size_t(k)] = T(val[L.perm[size_t(k)]]);
+            parallel_for(nnz, 1 << 16, [&](int64_t k) { h[size_t(k)] = T(val[L.perm[size_t(k)]]); });
         else
-            for (int64_t k = 0; k < nnz; ++k) h[size_t(k)] = T(val[k]);
+            parallel_for(nnz, 1 << 16, [&](int64_t k) { h[size_t(k)] = T(val[k]); });
         HIP_OK(hipMemcpyAsync(d, h.data(), size_t(nnz) * sizeof(T), hipMemcpyHostToDevice, st));
         HIP_OK(hipStreamSynchronize(st));
     }
