@@ -574,12 +574,14 @@ void sort_blocks(const std::vector<int32_t>& blk, const int32_t* ptr, const int3
 // slab's rows empty, and the slab layout then mostly stores, writes and adds empty partial sums (config 2's
 // matrices in 8 slabs: 76 and 37 us per product against 23 and 17 in one piece).
 bool slabs_pay(int rows, int cols, const int32_t* ptr, const int32_t* idx, int nslab) {
-    int64_t pairs = 0, nonempty = 0;
-    for (int i = 0; i < rows; ++i) {
+    std::vector<uint8_t> hits(static_cast<size_t>(rows));
+    parallel_for(rows, 8192, [&](int64_t i) {
         uint32_t hit = 0;
         for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) hit |= 1u << int(int64_t(idx[k]) * nslab / cols);
-        pairs += __builtin_popcount(hit), nonempty += ptr[i + 1] > ptr[i];
-    }
+        hits[size_t(i)] = uint8_t(__builtin_popcount(hit));
+    });
+    int64_t pairs = 0, nonempty = 0;
+    for (int i = 0; i < rows; ++i) pairs += hits[size_t(i)], nonempty += ptr[i + 1] > ptr[i];
     return 2 * pairs >= nonempty * nslab;
 }
 
@@ -618,24 +620,26 @@ bool slabs_pay(int rows, int cols, const int32_t* ptr, const int32_t* idx, int n
     const int NS = slab_count(cols);
     L.nslab = NS;
     auto slab_of = [&](int32_t c) { return int(int64_t(c) * NS / cols); };
+    // per slab: row lengths, then their prefix; row i's nonzeros of slab q go, in row order, to
+    // soff[q] + sp[q][i] ... (rows and slabs are independent, so these passes run on the host's cores)
     std::vector<std::vector<int32_t>> sp(NS, std::vector<int32_t>(size_t(rows) + 1, 0));
-    for (int i = 0; i < rows; ++i)
+    parallel_for(rows, 8192, [&](int64_t i) {
         for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) sp[size_t(slab_of(idx[k]))][size_t(i) + 1]++;
-    std::vector<int64_t> soff(NS + 1, 0);
-    for (int q = 0; q < NS; ++q) {
+    });
+    parallel_for(NS, 1, [&](int64_t q) {
         for (int i = 0; i < rows; ++i) sp[size_t(q)][size_t(i) + 1] += sp[size_t(q)][size_t(i)];
-        soff[size_t(q) + 1] = soff[size_t(q)] + sp[size_t(q)][size_t(rows)];
-    }
+    });
+    std::vector<int64_t> soff(NS + 1, 0);
+    for (int q = 0; q < NS; ++q) soff[size_t(q) + 1] = soff[size_t(q)] + sp[size_t(q)][size_t(rows)];
     L.idx.resize(size_t(nnz)), L.perm.resize(size_t(nnz));
-    std::vector<int64_t> next(NS * size_t(rows));
-    for (int q = 0; q < NS; ++q)
-        for (int i = 0; i < rows; ++i) next[size_t(q) * size_t(rows) + size_t(i)] = soff[size_t(q)] + sp[size_t(q)][size_t(i)];
-    for (int i = 0; i < rows; ++i)
+    parallel_for(rows, 8192, [&](int64_t i) {
+        int64_t at[2 * kSlabs];
+        for (int q = 0; q < NS; ++q) at[q] = soff[size_t(q)] + sp[size_t(q)][size_t(i)];
         for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-            const int q = slab_of(idx[k]);
-            const int64_t at = next[size_t(q) * size_t(rows) + size_t(i)]++;
-            L.idx[size_t(at)] = idx[k], L.perm[size_t(at)] = k;
+            const int64_t p = at[slab_of(idx[k])]++;
+            L.idx[size_t(p)] = idx[k], L.perm[size_t(p)] = k;
         }
+    });
     L.ptr.resize(size_t(NS) * (size_t(rows) + 1));
     L.snblk.resize(NS), L.sboff.resize(NS);
     // slab q holds the columns c with c * NS / cols == q, i.e. [ceil(q * cols / NS), ...)
@@ -651,14 +655,22 @@ bool slabs_pay(int rows, int cols, const int32_t* ptr, const int32_t* idx, int n
     std::vector<int64_t> perm2;
     if (L.sorted) {
         local.resize(size_t(nnz)), pk.resize(size_t(nnz)), perm2.resize(size_t(nnz));
-        for (int64_t k = 0; k < nnz; ++k) local[size_t(k)] = L.idx[size_t(k)] - L.sstart[size_t(slab_of(L.idx[size_t(k)]))];
+        parallel_for(nnz, 1 << 16, [&](int64_t k) {
+            local[size_t(k)] = L.idx[size_t(k)] - L.sstart[size_t(slab_of(L.idx[size_t(k)]))];
+        });
     }
-    for (int q = 0; q < NS; ++q) {
+    parallel_for(NS, 1, [&](int64_t q) {
         for (int i = 0; i <= rows; ++i)
             L.ptr[size_t(q) * (size_t(rows) + 1) + size_t(i)] = int32_t(soff[size_t(q)] + sp[size_t(q)][size_t(i)]);
+    });
+    std::vector<std::vector<int32_t>> bq(static_cast<size_t>(NS));  // each slab's blocks, cut and sorted on its own
+    parallel_for(NS, 1, [&](int64_t q) {
         const int32_t* qptr = L.ptr.data() + size_t(q) * (size_t(rows) + 1);
-        std::vector<int32_t> b = L.sorted ? sorted_blocks(rows, qptr, L.sb) : row_blocks(rows, sp[size_t(q)].data());
-        if (L.sorted) sort_blocks(b, qptr, local.data(), 0, L.perm.data(), L.cbits, pk, perm2);
+        bq[size_t(q)] = L.sorted ? sorted_blocks(rows, qptr, L.sb) : row_blocks(rows, sp[size_t(q)].data());
+        if (L.sorted) sort_blocks(bq[size_t(q)], qptr, local.data(), 0, L.perm.data(), L.cbits, pk, perm2);
+    });
+    for (int q = 0; q < NS; ++q) {
+        const std::vector<int32_t>& b = bq[size_t(q)];
         L.snblk[size_t(q)] = int(b.size()) - 1;
         L.sboff[size_t(q)] = int(L.blk.size());
         L.maxblk = std::max(L.maxblk, L.snblk[size_t(q)]);
