@@ -268,8 +268,8 @@ __global__ __launch_bounds__(WG) void k_sorted_slab(int rows, int nslab, const i
     block_row_sums<T, WG>(ptr, r0, r1, k0, sh, [&](int r, T s) { y[r] = s; });
 }
 
-// Slab layout, first kernel: workgroup b handles row block b / 8 of slab b % 8 (slab = XCD) and writes the
-// slab's partial row sums part[slab * rows + row]; same staging and reduction as k_rowblocks.
+// Slab layout, first kernel: workgroup b handles row block b / nslab of slab b % nslab (on XCD b % 8) and writes
+// the slab's partial row sums part[slab * rows + row]; same staging and reduction as k_rowblocks.
 template <class T>
 __global__ __launch_bounds__(kWG) void k_slab_partial(int rows, int nslab, const int* __restrict__ snblk,
                                                       const int* __restrict__ sboff, const int* __restrict__ blk_all,
@@ -461,8 +461,9 @@ struct Mem {  // device allocations owned by one backend
 
 // One CSR on the device in one of two layouts.
 //  plain: row blocks over the whole matrix (k_rowblocks).
-//  slab:  for a gathered vector of at least kSlabMinCols entries (half an XCD's L2 in FP32), the columns are
-//         split into nslab equal ranges (slab_count: 8 or 16) and the matrix stored slab-major as nslab CSR pieces
+//  slab:  for a gathered vector of at least kSlabMinCols entries (half an XCD's L2 in FP32) whose (row, slab)
+//         pairs are mostly occupied (slabs_pay), the columns are split into nslab equal ranges (slab_count: 8 or
+//         16) and the matrix stored slab-major as nslab CSR pieces
 //         with absolute row pointers. Workgroup b works on slab b % nslab, i.e. on XCD b % 8, so each XCD gathers
 //         from slices of the vector that stay in its own L2; each slab writes partial row sums and a combine
 //         kernel adds them in slab order and applies the epilogue.
