@@ -139,11 +139,13 @@ def roofline_entry(b_row, b_col, row_avg, col_avg, label=None):
                 col_gbs=round(b_col / (col_avg * 1e-3) / 1e9, 1) if col_avg else 0)
 
 
-def measured_traffic(workload, mode, kernel):
+def measured_traffic(workload, mode, kernel, side=None):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
-    this workload (copy-kernel calibrated), or None when there is none."""
+    this workload (copy-kernel calibrated), or None when there is none. `side` names the summary's line directly
+    (batches: "iter<stage>", one batched iteration at that stage); otherwise it is the kernel's step, row or col."""
     path = os.path.join(ROOT, "profiles", f"pdhg_{workload}_{mode}_traffic.md")
-    side = "col" if kernel.startswith("col") else "row"
+    if side is None:
+        side = "col" if kernel.startswith("col") else "row"
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -269,7 +271,8 @@ def run_batch(a, ffi, gen, desc, lps):
     achieved = b_it / (it_ms * 1e-3) / 1e9 if it_ms else 0.0
     roofline = dict(bound="hbm", kernel="one batched iteration (row + column launch, all instances live)",
                     achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=None, bytes_per_launch=int(b_it), avg_launch_ms=round(it_ms, 5))
+                    traffic=measured_traffic(a.workload, "fast", None, side=f"iter{info['cap']}"),
+                    bytes_per_launch=int(b_it), avg_launch_ms=round(it_ms, 5))
     errs = [abs(r["objective"] - md.planted_objective) / (1 + abs(md.planted_objective)) for r, md in zip(res, lps)]
     out = {
         "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
