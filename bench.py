@@ -2,30 +2,36 @@
 """Benchmark: LP solves/sec of the restarted PDHG solver on one MI355X, with the iteration-kernel roofline.
 
 Workloads (--workload):
-  cfg1 (default)  the single large planted LP: m = 1,000,000 equality rows, n = 2,000,000 columns, 20 nonzeros per
-                  column at uniform-random rows, N(0,1) values, 0 <= x <= 10, seed 1.
+  cfg3 (default)  a batch of 512 independent planted LPs of uneven size (m ~ U[500, 20000], n = round(m U[1.5, 3]),
+                  6-12 nonzeros per column per LP, seed 3), solved together by the batched fast path: the largest
+                  configuration of the set that one GPU holds and the batched path the >= 5x target names.
+  cfg1            the single large planted LP: m = 1,000,000 equality rows, n = 2,000,000 columns, 20 nonzeros per
+                  column at uniform-random rows, N(0,1) values, 0 <= x <= 10, seed 1 (one fast solve takes about
+                  two minutes).
   cfg2            the block-angular planning LP (64 blocks of 5,000 x 8,000 plus 200 linking rows), planted.
-  cfg3            a batch of 512 independent planted LPs of uneven size, solved together by the batched fast path.
-  cfg4            a scenario family of one planted LP (base 5,000 x 10,000 by default, --scenario-m): the learned
-                  warm start is trained on cold solves of 2,048 scenarios, then one step predicts and solves the
-                  512 held-out scenarios (batches of 512 sharing the matrix), with the cold-start fallback.
+  cfg4            a scenario family of one planted LP (--scenario-m rows, n = 2m): the learned warm start is trained
+                  on cold solves of --train-scenarios scenarios, then one step predicts and solves the held-out
+                  scenarios (batches sharing the matrix) with the cold-start fallback.
   cfg0, mid       the 2,000 x 5,000 and 100,000 x 200,000 planted LPs.
-Every solve runs to 1e-6 relative KKT (original-space primal residual, dual residual and gap, the reference's
-stopping test). A "step" is one full solve from a cold start of problems already resident in HBM (power iteration,
-PDHG iterations, FP64 evaluations, download and the original-space check); for cfg3 one step solves the whole
-batch and the value counts every LP. The default mode is the fast path (FP32 reflected Halpern PDHG iterates inside
-FP64 iterative refinement); one cfg1 solve takes minutes, so the defaults time one step without warm-up.
+Every solve runs to 1e-6 relative KKT (original-space primal residual, dual residual and gap: the reference's
+stopping test). A "step" is one full solve from a cold start (cfg4: the learned start) of problems already resident
+in HBM: power iteration, PDHG iterations, FP64 evaluations, download and the original-space check; for cfg3 one step
+solves the whole batch. `value` counts only solves that ended near_optimal.
+
+Wall-clock budget (--budget-s, from process start): the FP64 comparison and the CPU baseline run first, each capped;
+warm-up then stops early once it has used a fifth of what is left, and timed steps stop when the next one would
+not fit. `steps` and `warmup` report what ran (`steps_requested`, `warmup_requested` what was asked).
 
 Printed JSON line: the contract's fields, plus
-  roofline: single LPs: the dominant iteration kernel (A^T z fused with the primal step, or A x fused with the dual
-            step), algorithmic bytes per launch / mean launch time from HIP events on the solver stream in a
-            separate capped solve; batches: one batched iteration with every instance live (both launches),
-            timed by the batch's own stage autotuner; against the 8 TB/s HBM peak;
-  cpu_baseline: the CPU oracle (FP64, its iterations on up to 16 host threads partitioned as the reference's CPU
-            backend) timed on a bounded sample (about two seconds of PDHG iterations), scaled to solves/sec with
-            the GPU solve's iteration counts;
-  fp64_reference: the plain FP64 cold start of the reference algorithm (single LPs: the same LP, capped at
-            --fp64-time-limit; batches: the first --fp64-sample LPs one at a time).
+  roofline: the dominant iteration kernel's algorithmic bytes per launch / its mean launch time from HIP events on
+            the solver stream. Single LPs: a separate capped solve with an event pair around every launch. Batches:
+            `tpdhg_batch_probe`, all instances live, each row and column launch timed (a separately named kernel
+            instantiation, so a kernel trace of the same command reports exactly these launches);
+  cpu_baseline: the CPU oracle (FP64, the reference algorithm, its iteration block on the host threads partitioned
+            as the reference's CPU backend) timed on a bounded sample of PDHG iterations and extrapolated to whole
+            solves with the plain FP64 GPU solves' iteration counts (stated in `sample`);
+  fp64_reference: the plain FP64 cold start of the reference algorithm on the GPU (single LPs: the same LP; batches:
+            a stratified sample of the LPs, one at a time), each capped; both legs run the power iteration per solve.
 """
 import argparse
 import json
@@ -33,6 +39,7 @@ import os
 import sys
 import time
 
+T_START = time.perf_counter()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (ROOT, os.path.join(ROOT, "taral-lp_amd")):
     if p not in sys.path:
@@ -58,23 +65,16 @@ WORKLOADS = {  # name: (kind, description, generator)
                           "each rescaling a fixed 0.15% of b and of c by exp(theta)",
              lambda gen, a: gen.ScenarioFamily(gen.planted_lp(a.scenario_m, 2 * a.scenario_m, 6, seed=4), seed=4)),
 }
+DEFAULT_STEPS = {"cfg3": 3, "cfg1": 1, "cfg2": 3, "cfg4": 1, "cfg0": 5, "mid": 3}
 
 
 def kernel_bytes(m, n, nnz, s):
-    """Algorithmic HBM bytes of one launch of each fused iteration kernel (s = bytes per value):
-    row kernel: val+idx (s+4)/nnz, row pointers 4(m+1), x read once s*n, and per row z, Ax, rl, ru, Axs, zs
+    """Algorithmic HBM bytes of one launch of each fused iteration kernel of the reference scheme (s = bytes per
+    value): row kernel: val+idx (s+4)/nnz, row pointers 4(m+1), x read once s*n, and per row z, Ax, rl, ru, Axs, zs
     read and Ax, z, Axs, zs written (10 s per row); column kernel likewise over A^T with 10 s per column."""
     row = (s + 4) * nnz + 4 * (m + 1) + s * n + 10 * s * m
     col = (s + 4) * nnz + 4 * (n + 1) + s * m + 10 * s * n
     return row, col
-
-
-MODES = {  # name: solver options
-    "fast": dict(precision_bits=32, halpern=1, refine=1),   # FP32 reflected Halpern + FP64 refinement
-    "fp64": dict(precision_bits=64),                        # the reference algorithm, plain FP64 cold start
-    "fp32": dict(precision_bits=32),                        # the reference algorithm with FP32 iterates
-    "halpern-fp64": dict(precision_bits=64, halpern=1),
-}
 
 
 def halpern_kernel_bytes(m, n, nnz, s):
@@ -86,66 +86,124 @@ def halpern_kernel_bytes(m, n, nnz, s):
     return row, col
 
 
-def mode_options(mode):
-    if mode not in MODES:
-        raise SystemExit(f"unknown mode {mode}")
-    return dict(MODES[mode])
+MODES = {  # name: solver options
+    "fast": dict(precision_bits=32, halpern=1, refine=1),   # FP32 reflected Halpern + FP64 refinement
+    "fp64": dict(precision_bits=64),                        # the reference algorithm, plain FP64 cold start
+    "fp32": dict(precision_bits=32),                        # the reference algorithm with FP32 iterates
+    "halpern-fp64": dict(precision_bits=64, halpern=1),
+}
+DTYPES = {"fp64": "f64", "fp32": "f32 (f64 reductions)", "fast": "f32 iterates + f64 refinement",
+          "halpern-fp64": "f64"}
+CPU_THREADS = 16  # host threads for the CPU baseline: the CPU share of one GPU on the GPU hosts
 
 
-CPU_THREADS = 16  # host threads for the CPU baseline: 16 cores per GPU
+def log(msg):
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
+class Budget:
+    def __init__(self, total_s):
+        self.total = float(total_s)
+
+    def left(self):
+        return self.total - (time.perf_counter() - T_START)
 
 
 def cpu_threads():
     return max(1, min(CPU_THREADS, os.cpu_count() or 1))
 
 
-def cpu_baseline(md, iterations, sample_iters):
-    """Oracle FP64 on the host's cores (the reference CPU backend's row partition and barriers, one thread per
-    core, at most CPU_THREADS): time PDHG iterations of the same problem (at least `sample_iters`, doubled until the
-    sample takes two seconds of wall time), scale to one solve of `iterations` iterations."""
+def cpu_iter_cost(md, sample_s, threads):
+    """Seconds per PDHG iteration of the CPU oracle (FP64, the reference's iteration block on `threads` host threads
+    partitioned as its CPU backend) on `md`: iterations doubled until one sample takes `sample_s` seconds."""
     from oracle import oracle
-    nt = cpu_threads()
     o = oracle.OracleLP(md)
     t0 = time.perf_counter()
-    o.block(0.5, 0.5, 0, threads=nt)  # start: two products
+    o.block(0.5, 0.5, 0, threads=threads)  # start: two products
     t_start = time.perf_counter() - t0
-    k = max(1, sample_iters)
+    k = 2
     while True:
         t0 = time.perf_counter()
-        o.block(0.5, 0.5, k, threads=nt)
+        o.block(0.5, 0.5, k, threads=threads)
         t = time.perf_counter() - t0 - t_start
-        if t >= 2.0 or k >= 1 << 16:
-            break
+        if t >= sample_s or k >= 1 << 16:
+            return max(t, 1e-9) / k, k, t
         k *= 2
-    per_iter = max(t, 1e-9) / k
-    return dict(value=1.0 / (per_iter * iterations), unit="solves/s", cores=nt, kind="port",
-                sample=f"CPU oracle (oracle/pdhg_oracle.c, FP64, {nt} threads partitioned as the reference's "
-                       f"CPU backend): {k} PDHG iterations of the same LP took {t:.3f}s ({per_iter * 1e3:.3f} "
-                       f"ms/iter); scaled to the GPU solve's {iterations} iterations", ms_per_iter=per_iter * 1e3)
 
 
-def roofline_entry(b_row, b_col, row_avg, col_avg, label=None):
+def rel_kkt_of(r):
+    return max(r["rel_primal"], r["rel_dual"], r["rel_gap"])
+
+
+def solver_opts(a, mode_name, **extra):
+    o = dict(tol=a.tol, time_limit=a.time_limit, max_iter=a.max_iter, use_graph=0 if a.no_graph else 1,
+             verbose=int(a.verbose), progress_interval=a.progress, **MODES[mode_name])
+    if a.omega_rule is not None:
+        o["omega_rule"] = a.omega_rule
+    if a.reflection is not None and o.get("halpern"):
+        o["reflection"] = a.reflection
+    o.update(extra)
+    return o
+
+
+def timed_steps(step, steps, warmup, budget, count_ok):
+    """Warm-up (at most `warmup` steps, stopping once it has used a fifth of the budget left when it began), then up
+    to `steps` timed steps while the next one still fits the budget (at least one). `step()` returns its results;
+    `count_ok(results)` the near_optimal solves among them. Returns (wall, timed results, warm-ups run, solves ok)."""
+    w_done, longest = 0, 0.0
+    w_cap = max(0.0, budget.left()) / 5
+    t_w0 = time.perf_counter()
+    for w in range(warmup):
+        if w > 0 and time.perf_counter() - t_w0 + longest > w_cap:
+            break
+        t = time.perf_counter()
+        step()
+        longest = max(longest, time.perf_counter() - t)
+        w_done += 1
+        log(f"warm-up {w} done ({time.perf_counter() - t:.2f}s)")
+    results, ok = [], 0
+    t0 = time.perf_counter()
+    while len(results) < steps:
+        if results and budget.left() < 1.25 * longest:
+            log(f"budget: stopping after {len(results)} of {steps} timed steps")
+            break
+        t = time.perf_counter()
+        r = step()
+        longest = max(longest, time.perf_counter() - t)
+        results.append(r)
+        ok += count_ok(r)
+        log(f"timed step {len(results) - 1} done ({time.perf_counter() - t:.2f}s)")
+    return time.perf_counter() - t0, results, w_done, ok
+
+
+def contract_line(a, value, wall, n_steps, w_done, dtype, config):
+    return {"metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": n_steps,
+            "warmup": w_done, "steps_requested": a.steps, "warmup_requested": a.warmup,
+            "ms_per_step": round(1e3 * wall / max(n_steps, 1), 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def roofline_from(kind, row_bytes, col_bytes, t, traffic_of):
+    """Roofline entry of the dominant iteration kernel from HIP-event totals t (row/col ms and launch counts)."""
+    row_avg = t["row_ms"] / max(t["row_launches"], 1)
+    col_avg = t["col_ms"] / max(t["col_launches"], 1)
     if col_avg >= row_avg:
-        dom, dom_bytes, dom_avg = "col (A^T z + primal step)", b_col, col_avg
+        side, dom, b, avg = "col", "column step (A^T z fused with the primal step)", col_bytes, col_avg
     else:
-        dom, dom_bytes, dom_avg = "row (A x + dual step)", b_row, row_avg
-    if label:
-        dom = label
-    achieved = dom_bytes / (dom_avg * 1e-3) / 1e9 if dom_avg > 0 else 0.0
-    return dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(dom_bytes),
-                avg_launch_ms=round(dom_avg, 5), row_kernel_avg_ms=round(row_avg, 5), col_kernel_avg_ms=round(col_avg, 5),
-                row_gbs=round(b_row / (row_avg * 1e-3) / 1e9, 1) if row_avg else 0,
-                col_gbs=round(b_col / (col_avg * 1e-3) / 1e9, 1) if col_avg else 0)
+        side, dom, b, avg = "row", "row step (A x fused with the dual step)", row_bytes, row_avg
+    achieved = b / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+    return dict(bound="hbm", kernel=f"{kind}: {dom}", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_of(side), bytes_per_launch=int(b),
+                avg_launch_ms=round(avg, 5), launches=int(t[f"{side}_launches"]),
+                row_avg_ms=round(row_avg, 5), col_avg_ms=round(col_avg, 5),
+                row_gbs=round(row_bytes / (row_avg * 1e-3) / 1e9, 1) if row_avg else 0,
+                col_gbs=round(col_bytes / (col_avg * 1e-3) / 1e9, 1) if col_avg else 0)
 
 
-def measured_traffic(workload, mode, kernel, side=None):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
-    this workload (copy-kernel calibrated), or None when there is none. `side` names the summary's line directly
-    (batches: "iter<stage>", one batched iteration at that stage); otherwise it is the kernel's step, row or col."""
-    path = os.path.join(ROOT, "profiles", f"pdhg_{workload}_{mode}_traffic.md")
-    if side is None:
-        side = "col" if kernel.startswith("col") else "row"
+def measured_traffic(workload, side):
+    """HBM bytes per launch of the `side` ("row" or "col") kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"pdhg_{workload}_fast_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -156,162 +214,195 @@ def measured_traffic(workload, mode, kernel, side=None):
     return None
 
 
-def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def solver_opts(a, mode_name, **extra):
-    o = dict(tol=a.tol, time_limit=a.time_limit, max_iter=a.max_iter, use_graph=0 if a.no_graph else 1,
-             verbose=int(a.verbose), progress_interval=a.progress, **mode_options(mode_name))
-    if a.omega_rule is not None:
-        o["omega_rule"] = a.omega_rule
-    if a.reflection is not None and o.get("halpern"):
-        o["reflection"] = a.reflection
-    o.update(extra)
-    return o
-
-
-def rel_kkt_of(r):
-    return max(r["rel_primal"], r["rel_dual"], r["rel_gap"])
-
-
-def run_single(a, ffi, gen, desc, md):
-    model = ffi.Model.from_lp(md)
-    mode = a.mode
-
-    def timed(mode_name, steps, warmup, **extra):
-        t0 = time.perf_counter()
-        sess = ffi.Session(model, **solver_opts(a, mode_name, **extra))
-        setup = time.perf_counter() - t0
-        log(f"{mode_name}: session setup {setup:.1f}s")
-        for w in range(warmup):
-            r = sess.solve()
-            log(f"{mode_name} warmup {w}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s")
-        results = []
-        t0 = time.perf_counter()
-        for s_ in range(steps):
-            r = sess.solve()
-            results.append(r)
-            log(f"{mode_name} step {s_}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s")
-        wall = time.perf_counter() - t0
-        sess.close()
-        return wall, results, setup
-
-    wall, res, setup = timed(mode, a.steps, a.warmup)
-    last = res[-1]
-    value = a.steps / wall
-    # kernel roofline: a separate capped solve with HIP-event timing of every iteration launch (no graphs)
-    _, prof, _ = timed(mode, 1, 0, profile_kernels=1, max_iter=min(a.max_iter, a.profile_iters))
-    q = prof[-1]
-    s_val = 8 if MODES[mode]["precision_bits"] == 64 else 4
-    b_row, b_col = (halpern_kernel_bytes if MODES[mode].get("halpern") else kernel_bytes)(md.m, md.n, q["nnz"], s_val)
-    roofline = roofline_entry(b_row, b_col, q["row_kernel_ms"] / max(q["row_launches"], 1),
-                              q["col_kernel_ms"] / max(q["col_launches"], 1))
-    roofline["profiled_iterations"] = q["iterations"]
-    roofline["traffic"] = measured_traffic(a.workload, mode, roofline["kernel"])
-    out = {
-        "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[mode], "data": "synthetic",
-        "config": {"workload": desc, "m": md.m, "n": md.n, "nnz": last["nnz"], "tol": a.tol, "mode": mode,
-                   "parallelism": "single GPU"},
-        "iterations": last["iterations"], "restarts": last["restarts"], "refinements": last["refinements"],
-        "status": last["status"], "rel_kkt": rel_kkt_of(last),
-        "iters_per_s": round(last["iterations"] / last["solve_s"], 1), "session_setup_s": round(setup, 3),
-        "device": last["device"], "roofline": roofline,
-    }
-    if md.planted_objective is not None:
-        p_star = md.planted_objective
-        out["objective_rel_error_vs_planted"] = abs(last["objective"] - p_star) / (1 + abs(p_star))
-    if mode != "fp64" and not a.no_fp64_compare:
-        # the plain FP64 cold start of the reference algorithm, capped at --fp64-time-limit
-        saved = a.time_limit
-        a.time_limit = a.fp64_time_limit
-        w64, r64, _ = timed("fp64", 1, 0)
-        a.time_limit = saved
-        q = r64[-1]
-        out["fp64_reference"] = dict(status=q["status"], iterations=q["iterations"], wall_s=round(w64, 3),
-                                     rel_kkt=rel_kkt_of(q), time_cap_s=a.fp64_time_limit)
-        if q["status"] == "near_optimal":
-            out["fp64_solves_per_s"] = round(1 / w64, 6)
-            out["speedup_vs_fp64"] = round(value * w64, 4)
-    if not a.no_cpu_baseline:
-        log("cpu baseline sample")
-        out["cpu_baseline"] = cpu_baseline(md, last["iterations"], a.cpu_sample_iters)
+def fp64_sessions(a, ffi, models, label):
+    """Plain FP64 reference-algorithm solves, one model at a time, each on its own session (setup untimed), each
+    capped at --fp64-time-limit, the whole leg at --fp64-budget-s. Returns per-model (status, iterations, seconds)."""
+    out, t_leg = [], time.perf_counter()
+    for i, md in enumerate(models):
+        left = a.fp64_budget_s - (time.perf_counter() - t_leg)
+        if left <= 1:
+            log(f"fp64: leg budget spent after {i} of {len(models)}")
+            break
+        s = ffi.Session(md, **solver_opts(a, "fp64", time_limit=min(a.fp64_time_limit, left)))
+        t = time.perf_counter()
+        r = s.solve()
+        dt = time.perf_counter() - t
+        s.close()
+        out.append((r["status"], int(r["iterations"]), dt))
+        log(f"fp64 {label(i)}: {r['status']} it={r['iterations']} {dt:.2f}s")
     return out
 
 
-def run_batch(a, ffi, gen, desc, lps):
+def run_single(a, ffi, gen, desc, md, budget):
+    model = ffi.Model.from_lp(md)
+    mode = a.mode
+    out_extra = {}
+    # plain FP64 reference first (bounded), then the CPU sample, then the timed mode with what is left
+    fp64 = None
+    if mode != "fp64" and not a.no_fp64_compare:
+        fp64 = fp64_sessions(a, ffi, [model], lambda i: md.name)
+        if fp64:
+            st, it, dt = fp64[0]
+            out_extra["fp64_reference"] = dict(status=st, iterations=it, wall_s=round(dt, 3),
+                                               time_cap_s=a.fp64_time_limit, power_iteration="included")
+            if st == "near_optimal":
+                out_extra["fp64_solves_per_s"] = round(1 / dt, 6)
+    t = time.perf_counter()
+    sess = ffi.Session(model, **solver_opts(a, mode))
+    setup = time.perf_counter() - t
+    log(f"{mode}: session setup {setup:.1f}s")
+
+    def step():
+        r = sess.solve()
+        log(f"{mode}: {r['status']} it={r['iterations']} solve={r['solve_s']:.3f}s")
+        return r
+
+    wall, res, w_done, ok = timed_steps(step, a.steps, a.warmup, budget,
+                                        lambda r: int(r["status"] == "near_optimal"))
+    last = res[-1]
+    value = ok / wall
+    sess.close()
+    # kernel roofline: a separate capped solve with HIP-event timing of every iteration launch (no graphs)
+    sp = ffi.Session(model, **solver_opts(a, mode, profile_kernels=1, max_iter=min(a.max_iter, a.profile_iters)))
+    q = sp.solve()
+    sp.close()
+    s_val = 8 if MODES[mode]["precision_bits"] == 64 else 4
+    b_row, b_col = (halpern_kernel_bytes if MODES[mode].get("halpern") else kernel_bytes)(md.m, md.n, q["nnz"], s_val)
+    t = dict(row_ms=q["row_kernel_ms"], col_ms=q["col_kernel_ms"], row_launches=q["row_launches"],
+             col_launches=q["col_launches"])
+    roofline = roofline_from("one LP", b_row, b_col, t, lambda side: measured_traffic(a.workload, side))
+    config = {"workload": desc, "m": md.m, "n": md.n, "nnz": last["nnz"], "tol": a.tol, "mode": mode,
+              "parallelism": "single GPU"}
+    out = contract_line(a, value, wall, len(res), w_done, DTYPES[mode], config)
+    out.update({"solves_near_optimal": ok, "iterations": last["iterations"], "restarts": last["restarts"],
+                "refinements": last["refinements"], "status": last["status"], "rel_kkt": rel_kkt_of(last),
+                "iters_per_s": round(last["iterations"] / last["solve_s"], 1), "session_setup_s": round(setup, 3),
+                "power_iteration": "included in every timed solve", "device": last["device"],
+                "roofline": roofline})
+    out.update(out_extra)
+    if "fp64_solves_per_s" in out:
+        out["speedup_vs_fp64"] = round(value / out["fp64_solves_per_s"], 4)
+    if md.planted_objective is not None:
+        p_star = md.planted_objective
+        out["objective_rel_error_vs_planted"] = abs(last["objective"] - p_star) / (1 + abs(p_star))
+    if not a.no_cpu_baseline:
+        nt = cpu_threads()
+        per, k, ts = cpu_iter_cost(md, a.cpu_sample_s, nt)
+        use64 = fp64 and fp64[0][0] == "near_optimal"
+        its = fp64[0][1] if use64 else last["iterations"]
+        src = "the plain FP64 GPU solve's" if use64 else "the timed GPU solve's (the FP64 solve did not finish)"
+        out["cpu_baseline"] = dict(
+            value=1.0 / (per * its), unit="solves/s", cores=nt, kind="port", host_nproc=os.cpu_count(),
+            extrapolated=True, ms_per_iter=round(per * 1e3, 4),
+            sample=f"CPU oracle (oracle/pdhg_oracle.c, FP64, the reference algorithm's iteration block on {nt} host "
+                   f"threads partitioned as its CPU backend): {k} iterations of the same LP took {ts:.2f}s; "
+                   f"extrapolated to {its} iterations ({src} count)")
+    return out
+
+
+def stratified(nnzs, k, seed=7):
+    """k LP indices, one from each of k equal slices of the LPs sorted by nonzeros (seeded pick inside a slice)."""
+    import numpy as np
+    order = np.argsort(nnzs, kind="stable")
+    rng = np.random.default_rng(seed)
+    cuts = np.linspace(0, len(order), k + 1).astype(int)
+    return [int(order[rng.integers(cuts[i], cuts[i + 1])]) for i in range(k) if cuts[i + 1] > cuts[i]]
+
+
+def run_batch(a, ffi, gen, desc, lps, budget):
     import numpy as np
     if a.mode != "fast":
         raise SystemExit("batch workloads run the fast path (--mode fast)")
     models = [ffi.Model.from_lp(md) for md in lps]
+    nnzs = [md.nnz for md in lps]
+    k = len(lps) if a.fp64_sample <= 0 else min(a.fp64_sample, len(lps))
+    sample = list(range(len(lps))) if k == len(lps) else stratified(nnzs, k)
+    fp64 = None
+    if not a.no_fp64_compare:
+        fp64 = fp64_sessions(a, ffi, [models[i] for i in sample], lambda j: lps[sample[j]].name)
     t0 = time.perf_counter()
     batch = ffi.Batch(models, cap=a.cap, **solver_opts(a, "fast"))
     setup = time.perf_counter() - t0
     info = batch.info()
     log(f"batch of {len(lps)}: setup {setup:.1f}s, stage {info['cap']}, autotune ms {info['tune_ms']}")
-    for w in range(a.warmup):
-        batch.solve(points=False)
-    t0 = time.perf_counter()
-    for s_ in range(a.steps):
+    cpu = None
+    if not a.no_cpu_baseline:  # per-iteration CPU cost on the smallest, median and largest LP of the FP64 sample
+        nt = cpu_threads()
+        ref = sample if fp64 is None else sample[:len(fp64)]
+        by = sorted(ref, key=lambda i: nnzs[i])
+        picks = sorted({by[0], by[len(by) // 2], by[-1]})
+        costs = {}
+        for i in picks:
+            per, kk, ts = cpu_iter_cost(lps[i], a.cpu_sample_s / len(picks), nt)
+            costs[i] = per
+            log(f"cpu {lps[i].name}: {per * 1e3:.3f} ms/iter ({kk} iterations in {ts:.2f}s)")
+        cpu = (nt, costs)
+
+    def step():
         res = batch.solve(points=True)
-        log(f"batch step {s_}: {sum(r['status'] == 'near_optimal' for r in res)} of {len(res)} near_optimal")
-    wall = time.perf_counter() - t0
-    value = len(lps) * a.steps / wall
+        log(f"batch: {sum(r['status'] == 'near_optimal' for r in res)} of {len(res)} near_optimal")
+        return res
+
+    wall, runs, w_done, ok = timed_steps(step, a.steps, a.warmup, budget,
+                                        lambda res: sum(r["status"] == "near_optimal" for r in res))
+    res = runs[-1]
+    value = ok / wall
     its = np.array([r["iterations"] for r in res])
-    # roofline: one batched iteration with every instance live (row launch + column launch), from the autotune
-    # timing of the chosen stage size (HIP events around 24 iterations on the batch stream)
-    b_it = 0
-    for md in lps:
-        r_, c_ = halpern_kernel_bytes(md.m, md.n, md.nnz, 4)
-        b_it += r_ + c_
-    ti = {1024: 0, 2048: 1, 4096: 2, 8192: 3}.get(info["cap"])
-    it_ms = info["tune_ms"][ti] / 24 if ti is not None and info["tune_ms"][ti] > 0 else 0.0
-    achieved = b_it / (it_ms * 1e-3) / 1e9 if it_ms else 0.0
-    roofline = dict(bound="hbm", kernel="one batched iteration (row + column launch, all instances live)",
-                    achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic(a.workload, "fast", None, side=f"iter{info['cap']}"),
-                    bytes_per_launch=int(b_it), avg_launch_ms=round(it_ms, 5))
+    # roofline: one batched iteration with every instance live, both launches timed apart
+    t = batch.probe(a.probe_iters)
+    b_row = b_col = 0
+    for md, r in zip(lps, res):
+        r_, c_ = halpern_kernel_bytes(md.m, md.n, r["nnz"], 4)
+        b_row += r_
+        b_col += c_
+    roofline = roofline_from(f"{len(lps)} LPs all live, stage {info['cap']}", b_row, b_col, t,
+                             lambda side: measured_traffic(a.workload, side))
     errs = [abs(r["objective"] - md.planted_objective) / (1 + abs(md.planted_objective)) for r, md in zip(res, lps)]
-    out = {
-        "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES["fast"], "data": "synthetic",
-        "config": {"workload": f"{len(lps)} {desc}", "count": len(lps), "nnz_total": info["nnz_total"],
-                   "tol": a.tol, "mode": "fast (batched)", "stage": info["cap"], "parallelism": "single GPU"},
+    config = {"workload": f"{len(lps)} {desc}", "count": len(lps), "nnz_total": info["nnz_total"], "tol": a.tol,
+              "mode": "fast (batched)", "stage": info["cap"], "parallelism": "single GPU"}
+    out = contract_line(a, value, wall, len(runs), w_done, DTYPES["fast"], config)
+    out.update({
+        "solves_near_optimal": ok, "solves_attempted": len(lps) * len(runs),
         "status_counts": {s: sum(r["status"] == s for r in res) for s in sorted({r["status"] for r in res})},
-        "iterations": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max())},
+        "iterations": {"min": int(its.min()), "median": int(np.median(its)), "max": int(its.max()),
+                       "slowest": lps[int(its.argmax())].name},
         "max_rel_kkt": max(rel_kkt_of(r) for r in res), "max_objective_rel_error_vs_planted": max(errs),
-        "session_setup_s": round(setup, 3), "device": res[0]["device"], "roofline": roofline,
-    }
-    if not a.no_fp64_compare and a.fp64_sample > 0:
-        # the plain FP64 cold start of the reference algorithm, one LP at a time, on the first LPs of the batch
-        k = min(a.fp64_sample, len(lps))
-        t0 = time.perf_counter()
-        ok = 0
-        for i in range(k):
-            r = ffi.solve(models[i], **solver_opts(a, "fp64", time_limit=a.fp64_time_limit))
-            ok += r["status"] == "near_optimal"
-            log(f"fp64 {lps[i].name}: {r['status']} it={r['iterations']} {r['solve_s']:.2f}s")
-        t64 = time.perf_counter() - t0
-        out["fp64_reference"] = dict(sample=f"first {k} LPs of the batch, one at a time", converged=ok, wall_s=round(t64, 3),
-                                     solves_per_s=round(k / t64, 4), time_cap_s=a.fp64_time_limit)
-        out["speedup_vs_fp64"] = round(value / (k / t64), 3)
-    if not a.no_cpu_baseline:
-        log("cpu baseline sample")
-        i_med = int(np.argsort([md.nnz for md in lps])[len(lps) // 2])
-        cb = cpu_baseline(lps[i_med], max(int(its[i_med]), 1), a.cpu_sample_iters)
-        # CPU time of the whole batch: every LP's iterations at the sample's per-nonzero iteration cost
-        per_nnz = cb["ms_per_iter"] * 1e-3 / lps[i_med].nnz
-        total = sum(int(r["iterations"]) * md.nnz * per_nnz for r, md in zip(res, lps))
-        cb["value"] = len(lps) / total
-        cb["sample"] += f"; whole batch = sum of each LP's GPU iteration count x its nonzeros at that per-nonzero cost"
-        out["cpu_baseline"] = cb
+        "session_setup_s": round(setup, 3), "power_iteration": "included in every timed solve",
+        "device": res[0]["device"], "roofline": roofline})
+    if fp64:
+        done = [sample[j] for j in range(len(fp64))]
+        tot = sum(dt for _, _, dt in fp64)
+        conv = sum(st == "near_optimal" for st, _, _ in fp64)
+        rate = len(fp64) / tot
+        out["fp64_reference"] = dict(
+            sample=f"{len(fp64)} of {len(lps)} LPs, one from each nonzero-count slice (stratified, seed 7), solved "
+                   f"one at a time (each on its own resident session; power iteration included)",
+            lps=[lps[i].name for i in done], converged=conv, wall_s=round(tot, 3), solves_per_s=round(rate, 4),
+            time_cap_s=a.fp64_time_limit,
+            note="solves that hit the cap count at the cap, so the FP64 rate is an upper bound when converged < sample")
+        out["speedup_vs_fp64"] = round(value / rate, 3)
+    if cpu:
+        nt, costs = cpu
+        xs = np.array([nnzs[i] for i in costs], dtype=float)
+        ys = np.array([costs[i] for i in costs])
+        slope, icpt = np.polyfit(xs, ys, 1) if len(xs) > 1 else (ys[0] / xs[0], 0.0)
+        cost = lambda i: max(icpt + slope * nnzs[i], 1e-9)  # noqa: E731
+        if fp64:
+            est = sum(it * cost(sample[j]) for j, (_, it, _) in enumerate(fp64)) / len(fp64)
+            src = f"the plain FP64 GPU solves' iteration counts on the {len(fp64)} sampled LPs"
+        else:
+            est = float(np.mean([int(r["iterations"]) * cost(i) for i, r in enumerate(res)]))
+            src = "the fast-path GPU solves' iteration counts"
+        out["cpu_baseline"] = dict(
+            value=1.0 / est, unit="solves/s", cores=nt, kind="port", host_nproc=os.cpu_count(), extrapolated=True,
+            sample=f"CPU oracle (oracle/pdhg_oracle.c, FP64, the reference algorithm's iteration block on {nt} host "
+                   f"threads partitioned as its CPU backend): seconds per iteration timed on "
+                   f"{', '.join(lps[i].name for i in costs)} and fitted linearly in nonzeros; per-LP solve time = "
+                   f"that cost x {src}; value = 1 / mean per-LP time")
     return out
 
 
-def run_scenarios(a, ffi, gen, desc, fam):
+def run_scenarios(a, ffi, gen, desc, fam, budget):
     import numpy as np
     from taral_pdhg import learned
     if a.mode != "fast":
@@ -319,20 +410,35 @@ def run_scenarios(a, ffi, gen, desc, fam):
     th_tr, th_te = fam.thetas(a.train_scenarios, seed=1), fam.thetas(a.test_scenarios, seed=2)
     bs = min(a.scenario_batch, a.test_scenarios)
     extra = {} if a.omega_rule is None else {"omega_rule": a.omega_rule}
-    # training data: cold-start PDHG (the batched fast path) on the training scenarios
+    fp64 = None
+    if not a.no_fp64_compare and a.fp64_sample > 0:
+        # plain FP64 cold starts of held-out scenarios on one session of the base model (its norm reused, as the
+        # batched scenario solver reuses it)
+        s = ffi.Session(ffi.Model.from_lp(fam.base), **solver_opts(a, "fp64", reuse_norm=1))
+        fp64, t_leg = [], time.perf_counter()
+        for i in range(min(a.fp64_sample, len(th_te))):
+            left = a.fp64_budget_s - (time.perf_counter() - t_leg)
+            if left <= 1:
+                break
+            s.set_vectors(*fam.vectors(th_te[i]))
+            t = time.perf_counter()
+            r = s.solve()
+            fp64.append((r["status"], int(r["iterations"]), time.perf_counter() - t))
+            log(f"fp64 scenario {i}: {r['status']} it={r['iterations']} {fp64[-1][2]:.2f}s")
+        s.close()
     rtr, s_tr = learned.solve_family(fam, th_tr, batch_size=a.scenario_batch,
                                      log=lambda s_: log(f"training data: {s_}"), progress_interval=a.progress,
                                      **extra)
-    X = np.array([r["x"] for r in rtr]); Y = np.array([r["y"] for r in rtr])
+    X = np.array([r["x"] for r in rtr])
+    Y = np.array([r["y"] for r in rtr])
     W = np.array([r["primal_weight"] for r in rtr])
     wm = learned.WarmStartModel(fam, seed=1)
     fit = wm.fit(th_tr, X, Y, W, epochs=a.epochs)
     log(f"network trained on {len(th_tr)} scenarios: {fit}")
     solver = learned.ScenarioSolver(fam, bs, progress_interval=a.progress, warm_fallback=1, **extra)
-    info = solver.batch.info()
 
     def held_out(warm):
-        res, t0 = [], time.perf_counter()
+        res = []
         for i0 in range(0, len(th_te), bs):
             th = th_te[i0:i0 + bs]
             if warm:
@@ -340,36 +446,32 @@ def run_scenarios(a, ffi, gen, desc, fam):
                 res += solver.solve(th, p[0], p[1], p[2])
             else:
                 res += solver.solve(th)
-        return res, time.perf_counter() - t0
+        return res
 
-    cold, s_cold = held_out(False)
+    t = time.perf_counter()
+    cold = held_out(False)
+    s_cold = time.perf_counter() - t
     log(f"held-out cold: {len(th_te) / s_cold:.2f} solves/s")
-    for w in range(a.warmup):
-        held_out(True)
-    t0 = time.perf_counter()
-    for s_ in range(a.steps):
-        res, _ = held_out(True)
-        log(f"held-out learned step {s_}: {sum(r['status'] == 'near_optimal' for r in res)} near_optimal")
-    wall = time.perf_counter() - t0
-    value = len(th_te) * a.steps / wall
-    it_w = np.array([r["iterations"] for r in res]); it_c = np.array([r["iterations"] for r in cold])
-    # algorithmic bytes of one batched iteration: the shared matrix (A and A^T: values, indices, row pointers) once,
-    # every scenario's vectors
+    wall, runs, w_done, ok = timed_steps(lambda: held_out(True), a.steps, a.warmup, budget,
+                                        lambda res: sum(r["status"] == "near_optimal" for r in res))
+    res = runs[-1]
+    value = ok / wall
+    it_w = np.array([r["iterations"] for r in res])
+    it_c = np.array([r["iterations"] for r in cold])
+    t_probe = solver.batch.probe(a.probe_iters)
     m0, n0, z0 = fam.base.m, fam.base.n, fam.base.nnz
     r_, c_ = halpern_kernel_bytes(m0, n0, z0, 4)
-    b_mat = 2 * 8 * z0 + 4 * (m0 + 1) + 4 * (n0 + 1)
-    b_it = b_mat + bs * (r_ + c_ - b_mat)
-    ti = {1024: 0, 2048: 1, 4096: 2, 8192: 3}.get(info["cap"])
-    it_ms = info["tune_ms"][ti] / 24 if ti is not None and info["tune_ms"][ti] > 0 else 0.0
-    achieved = b_it / (it_ms * 1e-3) / 1e9 if it_ms else 0.0
-    out = {
-        "metric": METRIC, "value": round(value, 6), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES["fast"] + "; network bf16 (MFMA) / f32",
-        "data": "synthetic",
-        "config": {"workload": f"{desc}; base m={fam.base.m} n={fam.base.n}; {len(th_tr)} training and "
-                               f"{len(th_te)} held-out scenarios", "tol": a.tol,
-                   "mode": "learned warm start + fast path, batches of %d" % bs, "parallelism": "single GPU"},
+    # the shared matrix (values, indices, row pointers) is read once per launch; every scenario's vectors
+    b_row = (8 * z0 + 4 * (m0 + 1)) + bs * (r_ - 8 * z0 - 4 * (m0 + 1))
+    b_col = (8 * z0 + 4 * (n0 + 1)) + bs * (c_ - 8 * z0 - 4 * (n0 + 1))
+    roofline = roofline_from(f"{bs} scenarios all live (the shared matrix counted once)", b_row, b_col, t_probe,
+                             lambda side: None)
+    config = {"workload": f"{desc}; base m={fam.base.m} n={fam.base.n}; {len(th_tr)} training and {len(th_te)} "
+                          f"held-out scenarios", "tol": a.tol,
+              "mode": "learned warm start + fast path, batches of %d" % bs, "parallelism": "single GPU"}
+    out = contract_line(a, value, wall, len(runs), w_done, DTYPES["fast"] + "; network bf16 (MFMA) / f32", config)
+    out.update({
+        "solves_near_optimal": ok,
         "learned": {"training_solves_s": round(s_tr, 3), "train_s": round(fit["train_s"], 3),
                     "bases": fit["bases"], "explained_variance": round(fit["explained_variance"], 4),
                     "warm_used": int(sum(r["warm_start_used"] for r in res)),
@@ -380,54 +482,46 @@ def run_scenarios(a, ffi, gen, desc, fam):
         "iterations": {"learned_mean": float(it_w.mean()), "cold_mean": float(it_c.mean()),
                        "learned_max": int(it_w.max()), "cold_max": int(it_c.max())},
         "status_counts": {s2: sum(r["status"] == s2 for r in res) for s2 in sorted({r["status"] for r in res})},
-        "max_rel_kkt": max(rel_kkt_of(r) for r in res),
-        "roofline": dict(bound="hbm", kernel=f"one batched iteration of {bs} scenarios (row + column launch, all "
-                                             "live): the shared matrix counted once, every scenario's vectors",
-                         achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(b_it),
-                         avg_launch_ms=round(it_ms, 5)),
-    }
-    if not a.no_fp64_compare and a.fp64_sample > 0:
-        k = min(a.fp64_sample, len(th_te))
-        t0 = time.perf_counter()
-        ok = 0
-        for i in range(k):
-            r = ffi.solve(ffi.Model.from_lp(fam.model(th_te[i])), **solver_opts(a, "fp64", time_limit=a.fp64_time_limit))
-            ok += r["status"] == "near_optimal"
-            log(f"fp64 scenario {i}: {r['status']} it={r['iterations']} {r['solve_s']:.2f}s")
-        t64 = time.perf_counter() - t0
-        out["fp64_reference"] = dict(sample=f"first {k} held-out scenarios, one at a time, cold start", converged=ok,
-                                     wall_s=round(t64, 3), solves_per_s=round(k / t64, 4),
-                                     time_cap_s=a.fp64_time_limit)
-        out["speedup_vs_fp64"] = round(value / (k / t64), 3)
+        "max_rel_kkt": max(rel_kkt_of(r) for r in res), "roofline": roofline})
+    if fp64:
+        tot = sum(dt for _, _, dt in fp64)
+        out["fp64_reference"] = dict(sample=f"first {len(fp64)} held-out scenarios, one at a time, cold start",
+                                     converged=sum(st == "near_optimal" for st, _, _ in fp64), wall_s=round(tot, 3),
+                                     solves_per_s=round(len(fp64) / tot, 4), time_cap_s=a.fp64_time_limit)
+        out["speedup_vs_fp64"] = round(value / (len(fp64) / tot), 3)
     if not a.no_cpu_baseline:
-        log("cpu baseline sample")
-        out["cpu_baseline"] = cpu_baseline(fam.model(th_te[0]), max(int(it_c.mean()), 1), a.cpu_sample_iters)
-        out["cpu_baseline"]["sample"] += " (the held-out scenarios' mean cold iteration count)"
+        nt = cpu_threads()
+        per, k, ts = cpu_iter_cost(fam.model(th_te[0]), a.cpu_sample_s, nt)
+        its = int(np.mean([it for _, it, _ in fp64])) if fp64 else int(it_c.mean())
+        out["cpu_baseline"] = dict(
+            value=1.0 / (per * its), unit="solves/s", cores=nt, kind="port", host_nproc=os.cpu_count(),
+            extrapolated=True,
+            sample=f"CPU oracle (FP64, {nt} threads): {k} iterations of a held-out scenario took {ts:.2f}s; "
+                   f"extrapolated to {its} iterations (mean of the {'FP64 GPU solves' if fp64 else 'cold solves'})")
     return out
-
-
-DTYPES = {"fp64": "f64", "fp32": "f32 (f64 reductions)", "fast": "f32 iterates + f64 refinement",
-          "halpern-fp64": "f64"}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", default="cfg1", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg3", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="fast", choices=sorted(MODES))
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--budget-s", type=float, default=420.0,
+                    help="wall-clock budget of the whole run from process start (warm-up and timed steps stop in time)")
     ap.add_argument("--time-limit", type=float, default=600.0)
     ap.add_argument("--max-iter", type=int, default=2_000_000_000,
                     help="iteration cap per solve (profiling runs; the reported value then is not a full solve)")
     ap.add_argument("--profile-iters", type=int, default=4096,
-                    help="iterations of the separate HIP-event-timed solve behind the roofline figure")
+                    help="single LPs: iterations of the separate HIP-event-timed solve behind the roofline figure")
+    ap.add_argument("--probe-iters", type=int, default=24, help="batches: all-live iterations timed for the roofline")
     ap.add_argument("--no-fp64-compare", action="store_true")
-    ap.add_argument("--fp64-time-limit", type=float, default=120.0,
-                    help="time cap of each plain FP64 reference-algorithm solve reported beside the timed mode")
-    ap.add_argument("--fp64-sample", type=int, default=16, help="batch workloads: LPs solved one by one in FP64")
+    ap.add_argument("--fp64-time-limit", type=float, default=30.0, help="cap of each plain FP64 reference solve")
+    ap.add_argument("--fp64-budget-s", type=float, default=60.0, help="cap of the whole FP64 comparison leg")
+    ap.add_argument("--fp64-sample", type=int, default=16,
+                    help="batches: LPs solved one at a time in FP64 (stratified by nonzeros; 0 = all)")
     ap.add_argument("--batch-count", type=int, default=512, help="batch workloads: number of LPs")
     ap.add_argument("--cap", type=int, default=0, help="batch row-block stage (0 = autotune)")
     ap.add_argument("--reflection", type=float, default=None,
@@ -437,46 +531,40 @@ def main():
                          "default: the library's, 3)")
     ap.add_argument("--scenario-m", type=int, default=5000,
                     help="scenario workloads: rows of the base LP (n = 2m); the configuration's 50,000 needs ~1.5M "
-                         "iterations per scenario, so the default is a tenth of it")
+                         "iterations per scenario")
     ap.add_argument("--train-scenarios", type=int, default=2048)
     ap.add_argument("--test-scenarios", type=int, default=512)
     ap.add_argument("--scenario-batch", type=int, default=512)
     ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--verbose", action="store_true", help="solver progress on stderr every 50 evaluations")
     ap.add_argument("--no-graph", action="store_true",
-                    help="launch kernels directly instead of replaying HIP graphs (used under rocprofv3)")
+                    help="launch kernels directly instead of replaying HIP graphs")
     ap.add_argument("--progress", type=float, default=30.0,
                     help="seconds between the solver's progress lines on stderr (0 = none)")
-    ap.add_argument("--cpu-sample-iters", type=int, default=3)
+    ap.add_argument("--cpu-sample-s", type=float, default=12.0, help="seconds of CPU oracle iterations timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
-                    help="only time a plain device-to-device copy of this many GiB (3 times) and exit: the "
-                         "calibration kernel for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming read/copy ceiling measurement")
     a = ap.parse_args()
-    if a.copy_probe_gib > 0:
-        import torch
-        nbytes = int(a.copy_probe_gib * (1 << 30))
-        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / 3
-        print(json.dumps({"copy_bytes": nbytes, "ms": dt * 1e3, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}))
-        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this solver runs on one GPU; sharding the rows of A and A^T over several GPUs (with "
               "all-gathers of the iterate vectors) is not built", file=sys.stderr)
         sys.exit(2)
+    if a.steps is None:
+        a.steps = DEFAULT_STEPS[a.workload]
+    if a.steps < 1 or a.warmup < 0:
+        raise SystemExit("--steps must be >= 1 and --warmup >= 0")
+    budget = Budget(a.budget_s)
     from taral_pdhg import ffi, gen
     kind, desc, make = WORKLOADS[a.workload]
     t = time.perf_counter()
     data = make(gen, a)
     log(f"generated {a.workload} in {time.perf_counter() - t:.1f}s")
     run = {"single": run_single, "batch": run_batch, "scenarios": run_scenarios}[kind]
-    out = run(a, ffi, gen, desc, data)
+    out = run(a, ffi, gen, desc, data, budget)
+    if not a.no_ceiling:  # the box's streaming ceiling with the library's own read and copy kernels (1 GiB)
+        out["hbm_ceiling"] = {k: round(v, 1) for k, v in ffi.stream_bench(1 << 30, 10).items()}
+    out["budget_s"] = a.budget_s
+    out["bench_wall_s"] = round(time.perf_counter() - T_START, 1)
     print(json.dumps(out), flush=True)
 
 

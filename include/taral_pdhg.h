@@ -173,6 +173,9 @@ typedef struct {
                            smoothed update only; 1: kept within two decades of the starting weight; 2: residual
                            balancing (no move against a 10x primal/gap imbalance); 3 (default): both. The
                            reference scheme (halpern = 0) always uses the reference update */
+    int32_t reuse_norm; /* 1: a session or batch runs the power iteration on its first solve only and reuses ||A_s||
+                           afterwards (A_s never changes within one); 0 (default): every solve runs it, as each
+                           reference CLI call does (gpu/pdhg.cu:763) */
 } tpdhg_options;
 void tpdhg_options_default(tpdhg_options* o);
 
@@ -266,6 +269,12 @@ int tpdhg_batch_set_vectors(tpdhg_batch* b, int32_t i, const double* cost, const
  * results: count entries; x/y: NULL or count pointers (NULL entries skip that copy). */
 int tpdhg_batch_solve(tpdhg_batch* b, const double* const* x0, const double* const* y0, const double* primal_weight,
                       tpdhg_result* results, double* const* x, double* const* y);
+/* `iters` iterations of the whole batch with every instance live, each row and column launch timed by HIP events
+ * on the batch stream: total ms and launch counts per side (the roofline of one batched iteration). The probe
+ * launches are a separately named instantiation of the batched SpMV kernel, so kernel traces keep them apart from
+ * a solve's launches, whose live sets shrink. The next solve starts over from its base points. */
+int tpdhg_batch_probe(tpdhg_batch* b, int32_t iters, double* row_ms, double* col_ms, int64_t* row_launches,
+                      int64_t* col_launches);
 void tpdhg_batch_free(tpdhg_batch* b);
 
 /* ------------------------------------------------------------------ learned warm start (no reference counterpart) */
@@ -294,6 +303,11 @@ void tpdhg_mlp_destroy(tpdhg_mlp* net);
 int tpdhg_spmv_bench(int32_t precision_bits, int32_t rows, int32_t cols, const int32_t* ptr, const int32_t* idx,
                      const double* val, const double* v, double* y, int32_t reps, double* mean_ms,
                      double* median_ms);
+
+/* The device's streaming ceiling, measured with the library's own kernels: median GB/s of a read-only pass and
+ * of a copy (read + write bytes) over `bytes` bytes of HBM, `reps` timed launches each. The calibration kernel for
+ * the FETCH_SIZE / WRITE_SIZE counters and the achievable bound the SpMV kernels are compared with. */
+int tpdhg_stream_bench(int64_t bytes, int32_t reps, double* read_gbs, double* copy_gbs);
 
 #ifdef __cplusplus
 }

@@ -93,7 +93,8 @@ struct BEpiHCol {
 // block the nonzeros are in column order with their row-order position packed above the column; otherwise the
 // k_rowblocks scheme. A row longer than CAP is a block of its own, summed strided by the workgroup. With a list,
 // workgroup b takes block list[b] (nblk = list length), so a launch covers only the instances still iterating.
-template <class T, int CAP, bool SORTED, class Epi, int WG = stage_wg(CAP)>
+// TAG only names a second instantiation (the all-live probe launches) for kernel traces.
+template <class T, int CAP, bool SORTED, class Epi, int WG = stage_wg(CAP), int TAG = 0>
 __global__ __launch_bounds__(WG) void k_bblocks(int nblk, const int* __restrict__ list, const int* __restrict__ blk,
                                                  const int* __restrict__ pbase, const int* __restrict__ ptr,
                                                  const int* __restrict__ idx, const T* __restrict__ val,
@@ -568,6 +569,27 @@ public:
     const std::string& name() const override { return name_; }
     size_t mem_used() const override { return mem_used_; }
 
+    KernelTimes probe(int iters) override {
+        if (iters < 1) throw std::invalid_argument("probe needs at least one iteration");
+        std::vector<uint8_t> all(static_cast<size_t>(B_), 1);
+        HIP_OK(hipStreamSynchronize(st_));
+        collect_events();
+        const KernelTimes saved = times_;
+        upload_act(all);
+        ensure_lists(all);
+        launch_block(2, true);  // warm-up
+        HIP_OK(hipStreamSynchronize(st_));
+        collect_events();
+        times_ = KernelTimes();
+        launch_block(iters, true);
+        HIP_OK(hipStreamSynchronize(st_));
+        HIP_OK(hipGetLastError());
+        collect_events();
+        const KernelTimes t = times_;
+        times_ = saved;
+        return t;
+    }
+
 private:
     struct Side {  // device copy of one side's pool and the block table for the current stage size
         int rows_total = 0, nblk = 0;
@@ -852,14 +874,14 @@ private:
         HIP_OK(hipStreamSynchronize(st_));
     }
 
-    template <class S, class U, class Epi>
+    template <class S, class U, class Epi, int TAG = 0>
     void spmv_rows(const S& side, const U* v, const Epi& epi, const uint8_t* on, const int* list = nullptr,
                    int nlist = 0) {
         const int nb = list ? nlist : side.nblk;
         if (nb == 0) return;
 #define TPDHG_BB(CAPV, SRT)                                                                                      \
-    k_bblocks<U, CAPV, SRT, Epi><<<nb, stage_wg(CAPV), 0, st_>>>(nb, list, side.blk, side.pbase, side.ptr,        \
-                                                                 side.idx, side.val, side.voff, v, on, epi)
+    k_bblocks<U, CAPV, SRT, Epi, stage_wg(CAPV), TAG><<<nb, stage_wg(CAPV), 0, st_>>>(                          \
+        nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val, side.voff, v, on, epi)
         if (sorted_) {
             switch (cap_) {
                 case 1024: TPDHG_BB(1024, true); break;
@@ -923,11 +945,19 @@ private:
         }
         return events_[i];
     }
-    void launch_block(int steps) {
+    void launch_block(int steps, bool probe = false) {
         for (int s = 0; s < steps; ++s) {
             const BEpiHRow<T> er = row_epi(s);
             const BEpiHCol<T> ec = col_epi(s, 0, s + 1 == steps);
-            if (profile_) {
+            if (probe) {
+                const size_t e = pending_;
+                HIP_OK(hipEventRecord(event(e), st_));
+                spmv_rows<Side, T, BEpiHRow<T>, 1>(A_, w_, er, dact_, alist_, nalist_);
+                HIP_OK(hipEventRecord(event(e + 1), st_));
+                spmv_rows<Side, T, BEpiHCol<T>, 1>(AT_, z_, ec, dact_, tlist_, ntlist_);
+                HIP_OK(hipEventRecord(event(e + 2), st_));
+                pending_ += 3;
+            } else if (profile_) {
                 const size_t e = pending_;
                 HIP_OK(hipEventRecord(event(e), st_));
                 spmv_rows(A_, w_, er, dact_, alist_, nalist_);

@@ -52,6 +52,11 @@ public:
     virtual int cap() const = 0;  // row-block stage size in use
     // Autotune timings (ms for 24 iterations, stage sizes 1024, 2048, 4096, 8192; empty when the size was given).
     virtual std::vector<float> tune_ms() const = 0;
+    // `iters` iterations with every instance live, each row and column launch timed by HIP events on the batch
+    // stream. The launches use their own instantiation of the batched SpMV kernel (a distinct kernel name, same
+    // code), so a kernel trace separates these all-live launches from a solve's shrinking ones. Leaves the
+    // iterates changed: the next solve starts over from its base points.
+    virtual KernelTimes probe(int iters) = 0;
 };
 
 struct BatchOptions {

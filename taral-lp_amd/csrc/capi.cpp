@@ -81,6 +81,7 @@ tpdhg::Options to_options(const tpdhg_options* o) {
     r.halpern = o->halpern != 0, r.rho = o->reflection;
     r.omega0 = o->primal_weight, r.warm_fallback = o->warm_fallback != 0, r.progress_s = o->progress_interval;
     r.omega_rule = o->omega_rule;
+    r.reuse_norm = o->reuse_norm != 0;
     return r;
 }
 
@@ -364,6 +365,7 @@ void tpdhg_options_default(tpdhg_options* o) {
     o->halpern = d.halpern ? 1 : 0, o->reflection = d.rho;
     o->primal_weight = d.omega0, o->warm_fallback = d.warm_fallback ? 1 : 0, o->progress_interval = d.progress_s;
     o->omega_rule = d.omega_rule;
+    o->reuse_norm = d.reuse_norm ? 1 : 0;
 }
 
 namespace {
@@ -559,6 +561,17 @@ int tpdhg_batch_solve(tpdhg_batch* b, const double* const* x0, const double* con
     });
 }
 
+int tpdhg_batch_probe(tpdhg_batch* b, int32_t iters, double* row_ms, double* col_ms, int64_t* row_launches,
+                      int64_t* col_launches) {
+    if (!b || !row_ms || !col_ms || !row_launches || !col_launches) return fail(TPDHG_ERR_ARG, "null argument");
+    if (iters < 1 || iters > 100000) return fail(TPDHG_ERR_ARG, "iters must be in [1, 100000]");
+    return guarded([&]() -> int {
+        const tpdhg::KernelTimes t = b->s->device().probe(iters);
+        *row_ms = t.row_ms, *col_ms = t.col_ms, *row_launches = t.row_launches, *col_launches = t.col_launches;
+        return TPDHG_OK;
+    });
+}
+
 void tpdhg_batch_free(tpdhg_batch* b) { delete b; }
 
 int tpdhg_mlp_create(int32_t n_in, int32_t h1, int32_t h2, int32_t n_out, uint64_t seed, tpdhg_mlp** out) {
@@ -607,6 +620,15 @@ int tpdhg_spmv_bench(int32_t precision_bits, int32_t rows, int32_t cols, const i
     if (const char* e = csr_error(rows, cols, ptr, idx, val, -1)) return fail(TPDHG_ERR_ARG, std::string("bad CSR: ") + e);
     return guarded([&]() -> int {
         tpdhg::spmv_bench(precision_bits, rows, cols, ptr, idx, val, v, y, reps, mean_ms, median_ms);
+        return TPDHG_OK;
+    });
+}
+
+int tpdhg_stream_bench(int64_t bytes, int32_t reps, double* read_gbs, double* copy_gbs) {
+    if (!read_gbs || !copy_gbs || reps < 1 || reps > 1000 || bytes < (int64_t(1) << 20) || bytes > (int64_t(64) << 30))
+        return fail(TPDHG_ERR_ARG, "bytes must be in [1 MiB, 64 GiB], reps in [1, 1000], outputs non-null");
+    return guarded([&]() -> int {
+        tpdhg::stream_bench(bytes, reps, read_gbs, copy_gbs);
         return TPDHG_OK;
     });
 }

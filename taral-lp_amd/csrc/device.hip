@@ -500,6 +500,81 @@ void spmv_bench_t(int rows, int cols, const int32_t* ptr, const int32_t* idx, co
 }
 }  // namespace
 
+// Streaming ceiling of the device, measured with this library's own kernels: a read-only pass (each workgroup sums
+// its grid-stride share of 16-byte words, one float per workgroup out) and a 16-byte-word copy. Four independent
+// loads per thread per round keep enough requests in flight; the grid covers every CU several times.
+__global__ __launch_bounds__(kWG) void k_stream_read(int64_t n4, const float4* __restrict__ a, float* __restrict__ out) {
+    float s = 0;
+    const int64_t stride = int64_t(gridDim.x) * kWG;
+    int64_t i = int64_t(blockIdx.x) * kWG + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        s += (v0.x + v0.y + v0.z + v0.w) + (v1.x + v1.y + v1.z + v1.w) + (v2.x + v2.y + v2.z + v2.w) +
+             (v3.x + v3.y + v3.z + v3.w);
+    }
+    for (; i < n4; i += stride) {
+        const float4 v = a[i];
+        s += v.x + v.y + v.z + v.w;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ float red[kWG / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(kWG) void k_stream_copy(int64_t n4, const float4* __restrict__ a, float4* __restrict__ b) {
+    const int64_t stride = int64_t(gridDim.x) * kWG;
+    int64_t i = int64_t(blockIdx.x) * kWG + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        b[i] = v0, b[i + stride] = v1, b[i + 2 * stride] = v2, b[i + 3 * stride] = v3;
+    }
+    for (; i < n4; i += stride) b[i] = a[i];
+}
+
+void stream_bench(int64_t bytes, int reps, double* read_gbs, double* copy_gbs) {
+    require_device();
+    Mem mem;
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    const int64_t n4 = bytes / 16;
+    const int grid = 8192;  // 32 workgroups per CU on 256 CUs
+    float4* a = mem.alloc<float4>(size_t(n4));
+    float4* b = mem.alloc<float4>(size_t(n4));
+    float* out = mem.alloc<float>(size_t(grid));
+    HIP_OK(hipMemsetAsync(a, 0, size_t(n4) * 16, st));
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    auto median_ms = [&](auto&& launch) {
+        launch();  // warm-up
+        std::vector<float> ts;
+        for (int r = 0; r < reps; ++r) {
+            HIP_OK(hipEventRecord(e0, st));
+            launch();
+            HIP_OK(hipEventRecord(e1, st));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            ts.push_back(ms);
+        }
+        std::sort(ts.begin(), ts.end());
+        return double(ts[ts.size() / 2]);
+    };
+    const double tr = median_ms([&] { k_stream_read<<<grid, kWG, 0, st>>>(n4, a, out); });
+    const double tc = median_ms([&] { k_stream_copy<<<grid, kWG, 0, st>>>(n4, a, b); });
+    HIP_OK(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *read_gbs = double(n4) * 16 / (tr * 1e-3) / 1e9;
+    *copy_gbs = 2.0 * double(n4) * 16 / (tc * 1e-3) / 1e9;
+}
+
 void spmv_bench(int precision_bits, int rows, int cols, const int32_t* ptr, const int32_t* idx, const double* val,
                 const double* v, double* y, int reps, double* mean_ms, double* median_ms) {
     if (precision_bits == 32) spmv_bench_t<float>(rows, cols, ptr, idx, val, v, y, reps, mean_ms, median_ms);

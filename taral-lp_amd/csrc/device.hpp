@@ -118,6 +118,9 @@ int device_count();
 
 // Times y = M v for a CSR on the device with the solver's SpMV kernel: `reps` launches after one warm-up,
 // returns the mean and the median kernel time in ms (HIP events) and y (FP64).
+// Median HBM rates (GB/s) of this library's streaming-read and copy kernels over `bytes` bytes (copy: read +
+// write bytes), `reps` timed launches each.
+void stream_bench(int64_t bytes, int reps, double* read_gbs, double* copy_gbs);
 void spmv_bench(int precision_bits, int rows, int cols, const int32_t* ptr, const int32_t* idx, const double* val,
                 const double* v, double* y, int reps, double* mean_ms, double* median_ms);
 

@@ -341,10 +341,12 @@ Report Session::solve(const double* x0_orig, const double* y0_orig) {
         if (!(warm_kkt < cold_kkt)) warm = false, xw = zw = nullptr;
     }
     if (fast_)
-        rep.out = run_fast(*fast_, P, o_, o_.time_limit - setup_s_, xw, zw, warm ? o_.omega0 : 0, norm_a_);
+        rep.out = run_fast(*fast_, P, o_, o_.time_limit - setup_s_, xw, zw, warm ? o_.omega0 : 0,
+                           o_.reuse_norm ? norm_a_ : 0);
     else
-        rep.out = run_pdhg(*dev_, P, o_, o_.time_limit - setup_s_, xw, zw, warm ? o_.omega0 : 0, norm_a_);
-    norm_a_ = rep.out.norm_a;  // A_s is fixed for the session's lifetime: later solves skip the power iteration
+        rep.out = run_pdhg(*dev_, P, o_, o_.time_limit - setup_s_, xw, zw, warm ? o_.omega0 : 0,
+                           o_.reuse_norm ? norm_a_ : 0);
+    norm_a_ = rep.out.norm_a;  // A_s is fixed for the session's lifetime (reused only with reuse_norm)
     rep.out.warm_used = warm, rep.out.warm_kkt = warm_kkt, rep.out.cold_kkt = cold_kkt;
     rep.solve_s = since(t_call);
     finish_report(lp, P, rep);
@@ -595,6 +597,7 @@ std::vector<Report> BatchSession::solve(const std::vector<const double*>& x0, co
         if (warm[size_t(i)] && size_t(i) < omega0.size()) om[size_t(i)] = omega0[size_t(i)];
     std::vector<const Scaled*> ps;
     for (const Scaled& P : P_) ps.push_back(&P);
+    if (!o_.reuse_norm) norms_.clear();  // every solve runs the power iteration
     std::vector<Outcome> outs = run_batch(*dev_, ps, o_, o_.time_limit, xw, zw, om, &norms_);
     const double solve_s = since(t_call);
     std::vector<Report> reps(static_cast<size_t>(B));

@@ -50,6 +50,7 @@ struct Options {  // defaults of gpu/pdhg.cu:744-750 plus the options added here
     double progress_s = 0;       // > 0: a progress line on stderr every progress_s seconds of solving
     int omega_rule = 3;          // fast path and batches: primal-weight guard (driver.cpp:next_omega); the reference
                                  // scheme always uses the reference update
+    bool reuse_norm = false;     // sessions and batches: power iteration on the first solve only
 };
 
 struct Outcome {
@@ -94,7 +95,8 @@ struct Report {  // what the CLI prints and the C ABI returns (gpu/pdhg.cu:866-8
 };
 
 // A problem resident on the device: scaled once, uploaded once, then solved any number of times from cold or
-// warm starts. The first solve runs the power iteration; later solves reuse ||A_s|| (A_s never changes).
+// warm starts. Every solve runs the power iteration, unless Options::reuse_norm keeps the first solve's ||A_s||
+// (A_s never changes).
 class Session {
 public:
     Session(const LP& lp, const Options& o, Timings tm = Timings());

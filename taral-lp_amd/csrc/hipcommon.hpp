@@ -531,16 +531,19 @@ int sorted_block_size(int64_t nnz, int max_sb, int value_bytes) {
     return sb;
 }
 
-// Layout choice: TPDHG_LAYOUT=sorted or =unsorted forces one; otherwise ("auto") single matrices use the sorted
-// layout from kSortMinNnz nonzeros up (below that the solve is launch-bound and the row-block kernels measured
-// faster per iteration: 11.7 against 14.8 us on 10k-row LPs) and batches use it always.
-enum class LayoutMode { unsorted, sorted, automatic };
+// Layout choice: TPDHG_LAYOUT=sorted or =unsorted forces one; TPDHG_LAYOUT=slab forces the sorted slab layout on
+// every matrix of at least 2 * kSlabsK columns (whatever its size or slab occupancy: the production layout of the
+// largest matrices, made reachable at sizes a test solves in seconds); otherwise ("auto") single matrices use the
+// sorted layout from kSortMinNnz nonzeros up (below that the solve is launch-bound and the row-block kernels
+// measured faster per iteration: 11.7 against 14.8 us on 10k-row LPs) and batches use it always.
+enum class LayoutMode { unsorted, sorted, slab, automatic };
 constexpr int64_t kSortMinNnz = int64_t(1) << 20;
 
 LayoutMode layout_mode() {
     const char* e = std::getenv("TPDHG_LAYOUT");
     if (e && std::string(e) == "unsorted") return LayoutMode::unsorted;
     if (e && std::string(e) == "sorted") return LayoutMode::sorted;
+    if (e && std::string(e) == "slab") return LayoutMode::slab;
     return LayoutMode::automatic;
 }
 
@@ -593,9 +596,13 @@ bool slabs_pay(int rows, int cols, const int32_t* ptr, const int32_t* idx, int n
     if (nnz > kMaxNnz)
         throw std::length_error("a matrix of " + std::to_string(nnz) + " nonzeros exceeds the " +
                                 std::to_string(kMaxNnz) + " the kernels index with 32-bit integers");
-    L.slab = cols >= kSlabMinCols && rows > 0 && slabs_pay(rows, cols, ptr, idx, slab_count(cols));
     const LayoutMode mode = layout_mode();
-    const bool want_sorted = mode == LayoutMode::sorted || (mode == LayoutMode::automatic && nnz >= kSortMinNnz);
+    if (mode == LayoutMode::slab)
+        L.slab = rows > 0 && cols >= 2 * kSlabs;
+    else
+        L.slab = cols >= kSlabMinCols && rows > 0 && slabs_pay(rows, cols, ptr, idx, slab_count(cols));
+    const bool want_sorted = mode == LayoutMode::sorted || mode == LayoutMode::slab ||
+                             (mode == LayoutMode::automatic && nnz >= kSortMinNnz);
     if (!L.slab) {
         L.ptr.assign(ptr, ptr + rows + 1);
         if (want_sorted && cols <= (1 << 19)) {

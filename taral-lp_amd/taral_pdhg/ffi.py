@@ -34,9 +34,10 @@ EXPORTS = [
     "tpdhg_fast_download", "tpdhg_fast_destroy",
     "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
     "tpdhg_session_set_vectors", "tpdhg_session_set_primal_weight",
-    "tpdhg_result_json", "tpdhg_spmv_bench",
+    "tpdhg_result_json", "tpdhg_spmv_bench", "tpdhg_stream_bench",
     "tpdhg_recheck",
-    "tpdhg_batch_create", "tpdhg_batch_get_info", "tpdhg_batch_set_vectors", "tpdhg_batch_solve", "tpdhg_batch_free",
+    "tpdhg_batch_create", "tpdhg_batch_get_info", "tpdhg_batch_set_vectors", "tpdhg_batch_solve", "tpdhg_batch_probe",
+    "tpdhg_batch_free",
     "tpdhg_mlp_create", "tpdhg_mlp_param_count", "tpdhg_mlp_set_params", "tpdhg_mlp_get_params",
     "tpdhg_mlp_forward", "tpdhg_mlp_loss_grad", "tpdhg_mlp_train", "tpdhg_mlp_destroy",
 ]
@@ -72,7 +73,8 @@ class Options(C.Structure):
                 ("refine", C.c_int32), ("inner_tol", C.c_double), ("max_refinements", C.c_int32),
                 ("use_graph", C.c_int32), ("verbose", C.c_int32), ("profile_kernels", C.c_int32),
                 ("halpern", C.c_int32), ("reflection", C.c_double), ("primal_weight", C.c_double),
-                ("warm_fallback", C.c_int32), ("progress_interval", C.c_double), ("omega_rule", C.c_int32)]
+                ("warm_fallback", C.c_int32), ("progress_interval", C.c_double), ("omega_rule", C.c_int32),
+                ("reuse_norm", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -163,12 +165,14 @@ def lib():
         L.tpdhg_session_set_primal_weight.argtypes = [C.c_void_p, C.c_double]
         L.tpdhg_result_json.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result), C.c_char_p, C.c_int32]
         L.tpdhg_spmv_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d]
+        L.tpdhg_stream_bench.argtypes = [C.c_int64, C.c_int32, _d, _d]
         L.tpdhg_recheck.argtypes = [C.c_void_p, _d, _d, C.POINTER(RecheckResult)]
         _pp = C.POINTER(C.c_void_p)
         L.tpdhg_batch_create.argtypes = [_pp, C.c_int32, C.POINTER(Options), C.c_int32, C.POINTER(C.c_void_p)]
         L.tpdhg_batch_get_info.argtypes = [C.c_void_p, C.POINTER(BatchInfo)]
         L.tpdhg_batch_set_vectors.argtypes = [C.c_void_p, C.c_int32, _d, _d, _d, _d, _d]
         L.tpdhg_batch_solve.argtypes = [C.c_void_p, _pp, _pp, _d, C.POINTER(Result), _pp, _pp]
+        L.tpdhg_batch_probe.argtypes = [C.c_void_p, C.c_int32, _d, _d, _i64, _i64]
         L.tpdhg_batch_free.argtypes = [C.c_void_p]
         _f = C.POINTER(C.c_float)
         L.tpdhg_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]
@@ -535,6 +539,13 @@ class Batch:
         out["tune_ms"] = list(i.tune_ms)
         return out
 
+    def probe(self, iters=24) -> dict:
+        """HIP-event times of `iters` all-live batched iterations (row and column launches apart)."""
+        r, c = C.c_double(), C.c_double()
+        nr, nc = C.c_int64(), C.c_int64()
+        _check(lib().tpdhg_batch_probe(self._h, int(iters), C.byref(r), C.byref(c), C.byref(nr), C.byref(nc)))
+        return dict(row_ms=r.value, col_ms=c.value, row_launches=nr.value, col_launches=nc.value)
+
     def set_vectors(self, i, cost, col_lo, col_up, row_lo, row_up):
         keep = [_f64(v) for v in (cost, col_lo, col_up, row_lo, row_up)]
         _check(lib().tpdhg_batch_set_vectors(self._h, int(i), *[_p(v) for v in keep]))
@@ -652,3 +663,10 @@ def spmv_bench(ptr, idx, val, v, precision_bits=32, reps=20):
     _check(lib().tpdhg_spmv_bench(int(precision_bits), rows, v.size, _p(ptr, _i32), _p(idx, _i32), _p(val), _p(v),
                                   _p(y), int(reps), C.byref(mean), C.byref(med)))
     return y[:rows], mean.value, med.value
+
+
+def stream_bench(nbytes=1 << 30, reps=20) -> dict:
+    """Median GB/s of the library's streaming-read and copy kernels over nbytes of HBM."""
+    r, c = C.c_double(), C.c_double()
+    _check(lib().tpdhg_stream_bench(int(nbytes), int(reps), C.byref(r), C.byref(c)))
+    return dict(bytes=int(nbytes), read_gbs=r.value, copy_gbs=c.value)

@@ -127,11 +127,12 @@ def _g12(v):
 
 def _parsed(v):
     """What the MPS reader makes of a %.12g-formatted value (correctly rounded decimal to binary)."""
-    return np.array([float(t) for t in _g12(np.asarray(v, dtype=np.float64)).tolist()])
+    return np.array([float('%.12g' % t) for t in np.asarray(v, dtype=np.float64).tolist()], dtype=np.float64)
 
 
-def packing_mps(N: int, K: int = 5, seed: int = 1):
-    """(LPModel, MPS text) of ``gen_lp.py packing N --k K --seed seed`` (gpu/gen_lp.py:71-101)."""
+def packing_mps(N: int, K: int = 5, seed: int = 1, text: bool = True):
+    """(LPModel, MPS text) of ``gen_lp.py packing N --k K --seed seed`` (gpu/gen_lp.py:71-101); the text is None
+    with text=False (the model is the same: its values are those the text reads back as)."""
     rng = np.random.default_rng(seed)
     M = max(1, N // 2)
     rows = np.sort(rng.integers(0, M, (N, K)), axis=1)
@@ -153,6 +154,8 @@ def packing_mps(N: int, K: int = 5, seed: int = 1):
     model = LPModel(m=M, n=N, colptr=colptr, rowidx=rowidx, val=val, cost=_parsed(-p),
                     col_lo=np.zeros(N), col_up=np.ones(N), row_lo=np.full(M, -np.inf), row_up=_parsed(b),
                     name=f"PACKING_{N}")
+    if not text:
+        return model, None
     # MPS text, line for line as gpu/gen_lp.py:24-43 writes it
     rn = np.array([f'R{i}' for i in range(M)])
     out = [f'NAME PACKING_{N}\nROWS\n N OBJ\n', ''.join(f' L {r}\n' for r in rn), 'COLUMNS\n']

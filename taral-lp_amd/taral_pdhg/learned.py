@@ -20,7 +20,8 @@ import numpy as np
 
 from . import ffi, gen
 
-FAST = dict(tol=1e-6, precision_bits=32, halpern=1, refine=1)
+# the scenarios of a family share A, so the batch keeps ||A_s|| from its first solve (reuse_norm)
+FAST = dict(tol=1e-6, precision_bits=32, halpern=1, refine=1, reuse_norm=1)
 
 
 class ScenarioSolver:

@@ -200,6 +200,22 @@ static void run(const Case& c, bool sorted_mode, unsigned seed) {
 }
 
 int main(int argc, char** argv) {
+    if (argc > 1 && std::strcmp(argv[1], "slab") == 0) {  // TPDHG_LAYOUT=slab: sorted slabs on small matrices too
+        setenv("TPDHG_LAYOUT", "slab", 1);
+        unsetenv("TPDHG_SLABS");
+        const Case forced[] = {
+            {"forced_plain", 3000, 5000, 10, 0, 0, 1, 8},
+            {"forced_long_rows", 5000, 4000, 30, 3, 3900, 1, 8},
+            {"forced_tiny_one_piece", 1, 10, 3, 0, 0, 1, 0},  // fewer than 16 columns: no slabs
+            {"forced_block_structured", 16000, 110000, 16, 10, 10240, 64, 8},
+            {"forced_runs_of_empty_rows", 12000, 60000, 20, 0, 0, 1, 8, 1500},
+            {"forced_wide", 2000, 1200000, 30, 0, 0, 1, 16},
+        };
+        unsigned seed = 101;
+        for (const Case& c : forced) run(c, c.want_slabs > 0, seed++);
+        std::printf("fails %d\n", fails);
+        return fails != 0;
+    }
     const bool sorted_mode = argc > 1 && std::strcmp(argv[1], "sorted") == 0;
     setenv("TPDHG_LAYOUT", sorted_mode ? "sorted" : "unsorted", 1);
     unsetenv("TPDHG_SLABS");

@@ -52,3 +52,15 @@ def test_batch_list(packing_10000, transport_100x100, tmp_path):
     # may differ in summation order; both stop at 1e-6 relative KKT)
     single = ffi.solve(ffi.Model.from_mps(b), tol=1e-6, precision_bits=32, halpern=1, refine=1)
     assert rows[1]["objective"] == pytest.approx(single["objective"], rel=1e-5)
+
+
+def test_row_rel_flag(packing_10000, tmp_path):
+    """`--row-rel 1` on the CLI takes the per-row stop test's longer trajectory (the oracle's 6,080 iterations and 9
+    restarts at 1e-4 on packing_10000, tests/test_oracle.py) instead of the default 640 / 4."""
+    mps = _write(tmp_path, "p.mps", packing_10000[1])
+    js = str(tmp_path / "p.json")
+    r = subprocess.run([ffi.CLI_PATH, mps, "--device", "gpu", "--tol", "1e-4", "--row-rel", "1", "--json", js],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = json.load(open(js))
+    assert out["status"] == "near_optimal" and (out["iterations"], out["restarts"]) == (6080, 9)

@@ -20,9 +20,10 @@ from taral_pdhg import ffi, gen
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["sorted", "unsorted"], autouse=True)
+@pytest.fixture(params=["sorted", "unsorted", "slab"], autouse=True)
 def layout(request, monkeypatch):
-    """Every fast-path check on both SpMV layouts (the small test problems default to row blocks)."""
+    """Every fast-path check on the three SpMV layouts (the small test problems default to row blocks; "slab"
+    forces the sorted column slabs with their combine epilogues, the layout of the largest matrices)."""
     monkeypatch.setenv("TPDHG_LAYOUT", request.param)
     return request.param
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")

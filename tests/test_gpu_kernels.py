@@ -47,11 +47,12 @@ SPMV_CASES = {  # row lengths; gathered vectors of 2^19 entries or more take the
 }
 
 
-@pytest.mark.parametrize("layout", ["sorted", "unsorted"])
+@pytest.mark.parametrize("layout", ["sorted", "unsorted", "slab"])
 @pytest.mark.parametrize("bits", [64, 32])
 @pytest.mark.parametrize("case", sorted(SPMV_CASES))
 def test_spmv_parity(case, bits, layout, monkeypatch):
-    monkeypatch.setenv("TPDHG_LAYOUT", layout)  # both layouts: column-sorted blocks and row blocks
+    # column-sorted blocks, row blocks, and sorted column slabs forced on every size ("slab")
+    monkeypatch.setenv("TPDHG_LAYOUT", layout)
     rng = np.random.default_rng(11)
     lengths = SPMV_CASES[case](rng)
     cols = 600_000 if case.startswith("slab") else 12000
@@ -153,10 +154,18 @@ def _close(a, b, rel=1e-10):
     assert np.max(np.abs(np.asarray(a) - np.asarray(b))) <= rel * scale
 
 
+@pytest.mark.parametrize("layout", ["auto", "sorted", "slab"])
 @pytest.mark.parametrize("which", ["packing", "cfg0"])
-def test_backend_block_fp64(which, packing_problem, cfg0_problem):
+def test_backend_block_fp64(which, layout, packing_problem, cfg0_problem, monkeypatch):
+    """One 64-step block of the reference scheme on each layout: row blocks (the default at these sizes), column-
+    sorted blocks (k_sorted with EpiRow / EpiCol) and sorted slabs (k_sorted_slab + k_combine_epi with EpiRow /
+    EpiCol), each against the oracle's block."""
+    if layout == "auto":
+        monkeypatch.delenv("TPDHG_LAYOUT", raising=False)
+    else:
+        monkeypatch.setenv("TPDHG_LAYOUT", layout)
     md, S, O = packing_problem if which == "packing" else cfg0_problem
-    print(f"backend block fp64 {which}", flush=True)
+    print(f"backend block fp64 {which} layout={layout}", flush=True)
     dev = ffi.Device(S, precision_bits=64, eval_freq=64, use_graph=True)
     norm = dev.power_norm(200)
     assert norm == pytest.approx(O.power_norm(fp32=False), rel=1e-10)

@@ -24,7 +24,7 @@ def layout_check(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("mode", ["sorted", "unsorted"])
+@pytest.mark.parametrize("mode", ["sorted", "unsorted", "slab"])
 def test_layouts_replayed_on_cpu(layout_check, mode):
     env = {k: v for k, v in os.environ.items() if k not in ("TPDHG_SLABS", "TPDHG_LAYOUT")}
     r = subprocess.run([layout_check, mode], capture_output=True, text=True, timeout=600, env=env)

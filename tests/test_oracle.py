@@ -52,6 +52,39 @@ def test_oracle_packing_100000_ledger():
     assert float("%.12g" % r["objective"]) == case["objective"]
 
 
+def test_oracle_transport_316x316_ledger():
+    """synth_transport_tol1e-4.csv:7: 31,616 iterations, 21 restarts (about 20 s on one core)."""
+    case = next(c for c in LEDGER["cases"] if c["case"] == "transport_316x316")
+    md = _instance(case)
+    assert (md.m, md.n, md.nnz) == (case["rows"], case["cols"], case["nnz"])
+    r = oracle.OracleLP(md).solve(tol=case["tol"])
+    assert (r["iterations"], r["restarts"]) == (case["iterations"], case["restarts"])
+    assert float("%.12g" % r["objective"]) == case["objective"]
+
+
+def _row_rel_norm(md, x):
+    """||row violation_i / (1 + |b_i|)||_2 on the original model, b_i the larger finite |row bound|
+    (gpu/pdhg.cu:137-139, 748, 793)."""
+    ax = md.matvec(x)
+    viol = np.maximum(md.row_lo - ax, 0) + np.maximum(ax - md.row_up, 0)
+    lo, up = md.row_lo, md.row_up
+    b = np.where(np.isfinite(lo) & np.isfinite(up), np.maximum(abs(lo), abs(up)),
+                 np.where(np.isfinite(lo), abs(lo), np.where(np.isfinite(up), abs(up), 0.0)))
+    return float(np.linalg.norm(viol / (1 + b)))
+
+
+@pytest.mark.parametrize("case", ["packing_10000", "transport_100x100"])
+def test_oracle_row_rel_stop_test(case):
+    """--row-rel K adds the per-row test ||viol_i / (1 + |b_i|)|| <= K tol: off, the returned point breaks it at
+    K = 1 (so the test binds on these LPs); on, the returned point meets it."""
+    md = gen.packing_mps(10000)[0] if case.startswith("packing") else gen.transport_mps(100, 100)[0]
+    O = oracle.OracleLP(md)
+    off, on = O.solve(tol=1e-4), O.solve(tol=1e-4, row_rel=1.0)
+    assert _row_rel_norm(md, off["x"] * O.vec("Dc")) > 1e-4
+    assert _row_rel_norm(md, on["x"] * O.vec("Dc")) <= 1e-4
+    assert on["status"] == "near_optimal" and on["iterations"] > off["iterations"]
+
+
 def test_planted_optimum_and_dual_simplex(cfg0, golden_dir):
     g = json.load(open(os.path.join(golden_dir, "cfg0_dual_simplex.json")))
     p_star = cfg0.planted_objective
