@@ -304,6 +304,20 @@ int tpdhg_spmv_bench(int32_t precision_bits, int32_t rows, int32_t cols, const i
                      const double* val, const double* v, double* y, int32_t reps, double* mean_ms,
                      double* median_ms);
 
+/* ------------------------------------------------------------------ allocation-free SpMV */
+/* y = M v on caller-owned DEVICE arrays, enqueued on `stream` (a hipStream_t; NULL = the default stream) and
+ * returned without waiting. No reference counterpart: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops) and calls no hipMalloc, hipFree or allocating routine. ptr
+ * (rows + 1) and idx (nnz) are int32 CSR with ptr[rows] == nnz and column indices in range (not checked: the
+ * arrays stay on the device); val, v, y are float (precision_bits 32, FP32 accumulation) or double (64). The
+ * partial sums of rows cut by the 2,048-nonzero work segments live in one static device array of 1 MiB per
+ * device, so nnz is limited to tpdhg_spmv_noalloc_max_nnz() (134,217,728); larger inputs are refused with
+ * TPDHG_ERR_ARG. Calls that overlap on one device share that array and race: order them on one stream. The same
+ * product as tpdhg_spmv_bench, summed in another order. */
+int tpdhg_spmv_noalloc(int32_t precision_bits, int32_t rows, int64_t nnz, const int32_t* ptr, const int32_t* idx,
+                       const void* val, const void* v, void* y, void* stream);
+int64_t tpdhg_spmv_noalloc_max_nnz(void);
+
 /* The device's streaming ceiling, measured with the library's own kernels: median GB/s of a read-only pass and
  * of a copy (read + write bytes) over `bytes` bytes of HBM, `reps` timed launches each. The calibration kernel for
  * the FETCH_SIZE / WRITE_SIZE counters and the achievable bound the SpMV kernels are compared with. */

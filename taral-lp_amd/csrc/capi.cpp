@@ -9,6 +9,7 @@
 
 #include "../../include/taral_pdhg.h"
 #include "driver.hpp"
+#include "noalloc.hpp"
 #include "mlp.hpp"
 #include "recheck.hpp"
 
@@ -623,6 +624,23 @@ int tpdhg_spmv_bench(int32_t precision_bits, int32_t rows, int32_t cols, const i
         return TPDHG_OK;
     });
 }
+
+int tpdhg_spmv_noalloc(int32_t precision_bits, int32_t rows, int64_t nnz, const int32_t* ptr, const int32_t* idx,
+                       const void* val, const void* v, void* y, void* stream) {
+    if (precision_bits != 32 && precision_bits != 64) return fail(TPDHG_ERR_ARG, "precision must be 32 or 64 bits");
+    if (rows < 0 || nnz < 0) return fail(TPDHG_ERR_ARG, "rows and nnz must be >= 0");
+    if (nnz > tpdhg::spmv_noalloc_max_nnz())
+        return fail(TPDHG_ERR_ARG, "nnz exceeds the " + std::to_string(tpdhg::spmv_noalloc_max_nnz()) +
+                                       " nonzeros the allocation-free SpMV's static scratch array covers");
+    if (rows > 0 && (!ptr || !y || (nnz > 0 && (!idx || !val || !v))))
+        return fail(TPDHG_ERR_ARG, "null device array");
+    return guarded([&]() -> int {
+        tpdhg::spmv_noalloc(precision_bits, rows, nnz, ptr, idx, val, v, y, static_cast<hipStream_t>(stream));
+        return TPDHG_OK;
+    });
+}
+
+int64_t tpdhg_spmv_noalloc_max_nnz(void) { return tpdhg::spmv_noalloc_max_nnz(); }
 
 int tpdhg_stream_bench(int64_t bytes, int32_t reps, double* read_gbs, double* copy_gbs) {
     if (!read_gbs || !copy_gbs || reps < 1 || reps > 1000 || bytes < (int64_t(1) << 20) || bytes > (int64_t(64) << 30))

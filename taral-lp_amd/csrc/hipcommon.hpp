@@ -803,7 +803,7 @@ void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st)
     return nm;
 }
 
-void require_device() {
+[[maybe_unused]] void require_device() {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
         throw std::runtime_error("no HIP device visible: the PDHG solver runs on an AMD GPU only (there is no CPU path)");

@@ -34,7 +34,7 @@ EXPORTS = [
     "tpdhg_fast_download", "tpdhg_fast_destroy",
     "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
     "tpdhg_session_set_vectors", "tpdhg_session_set_primal_weight",
-    "tpdhg_result_json", "tpdhg_spmv_bench", "tpdhg_stream_bench",
+    "tpdhg_result_json", "tpdhg_spmv_bench", "tpdhg_stream_bench", "tpdhg_spmv_noalloc", "tpdhg_spmv_noalloc_max_nnz",
     "tpdhg_recheck",
     "tpdhg_batch_create", "tpdhg_batch_get_info", "tpdhg_batch_set_vectors", "tpdhg_batch_solve", "tpdhg_batch_probe",
     "tpdhg_batch_free",
@@ -166,6 +166,9 @@ def lib():
         L.tpdhg_result_json.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result), C.c_char_p, C.c_int32]
         L.tpdhg_spmv_bench.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32, _i32, _d, _d, _d, C.c_int32, _d, _d]
         L.tpdhg_stream_bench.argtypes = [C.c_int64, C.c_int32, _d, _d]
+        L.tpdhg_spmv_noalloc.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tpdhg_spmv_noalloc_max_nnz.restype = C.c_int64
         L.tpdhg_recheck.argtypes = [C.c_void_p, _d, _d, C.POINTER(RecheckResult)]
         _pp = C.POINTER(C.c_void_p)
         L.tpdhg_batch_create.argtypes = [_pp, C.c_int32, C.POINTER(Options), C.c_int32, C.POINTER(C.c_void_p)]
@@ -670,3 +673,10 @@ def stream_bench(nbytes=1 << 30, reps=20) -> dict:
     r, c = C.c_double(), C.c_double()
     _check(lib().tpdhg_stream_bench(int(nbytes), int(reps), C.byref(r), C.byref(c)))
     return dict(bytes=int(nbytes), read_gbs=r.value, copy_gbs=c.value)
+
+
+def spmv_noalloc(precision_bits, rows, nnz, ptr, idx, val, v, y, stream=0):
+    """The allocation-free SpMV on device pointers (integers: e.g. torch tensors' data_ptr()), enqueued on the
+    HIP stream handle `stream` (0 = the default stream); returns without waiting."""
+    _check(lib().tpdhg_spmv_noalloc(int(precision_bits), int(rows), int(nnz), ptr, idx, val, v, y,
+                                    C.c_void_p(stream) if stream else None))
