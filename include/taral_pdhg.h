@@ -256,6 +256,9 @@ typedef struct {
     double setup_s;           /* scaling, upload and stage-size autotuning */
     float tune_ms[4];         /* autotune timings (ms, 24 iterations) for 1024 / 2048 / 4096 / 8192; 0 when cap
                                  was given */
+    int32_t persist_max;      /* live row/column blocks up to which an iteration block runs as one persistent
+                                 launch (0: off, or switched off after a failed launch) */
+    int64_t persistent_blocks, lockstep_blocks; /* iteration blocks run each way so far */
 } tpdhg_batch_info;
 /* opt->halpern must be 1. cap: 0 = pick the row-block stage by timing, else 1024, 2048, 4096 or 8192. The batch keeps
  * copies of the models. */

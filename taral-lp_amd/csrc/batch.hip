@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <map>
 #include <numeric>
 
@@ -91,19 +92,14 @@ struct BEpiHCol {
 // Row-block SpMV over the batch with a CAP-entry LDS stage. blk holds (instance, first row, end row) triples;
 // rows are local to the instance. SORTED: the sorted layout of hipcommon.hpp (k_sorted) per instance: within a
 // block the nonzeros are in column order with their row-order position packed above the column; otherwise the
-// k_rowblocks scheme. A row longer than CAP is a block of its own, summed strided by the workgroup. With a list,
-// workgroup b takes block list[b] (nblk = list length), so a launch covers only the instances still iterating.
-// TAG only names a second instantiation (the all-live probe launches) for kernel traces.
-template <class T, int CAP, bool SORTED, class Epi, int WG = stage_wg(CAP), int TAG = 0>
-__global__ __launch_bounds__(WG) void k_bblocks(int nblk, const int* __restrict__ list, const int* __restrict__ blk,
-                                                 const int* __restrict__ pbase, const int* __restrict__ ptr,
-                                                 const int* __restrict__ idx, const T* __restrict__ val,
-                                                 const int* __restrict__ voff, const T* __restrict__ v,
-                                                 const uint8_t* __restrict__ on, Epi epi) {
-    __shared__ T sh[CAP];
-    __shared__ T red[WG / 64];
-    const int lb = xcd_block(int(blockIdx.x), nblk);
-    const int b = list ? list[lb] : lb;  // list: the live instances' blocks, in instance order
+// k_rowblocks scheme. A row longer than CAP is a block of its own, summed strided by the workgroup. One workgroup
+// of WG threads runs block b; sh (CAP values) and red (WG / 64) are its LDS. The whole workgroup takes the same
+// path, so the barriers inside are uniform.
+template <class T, int CAP, bool SORTED, int WG, class Epi>
+__device__ __forceinline__ void bblock(int b, const int* __restrict__ blk, const int* __restrict__ pbase,
+                                       const int* __restrict__ ptr, const int* __restrict__ idx,
+                                       const T* __restrict__ val, const int* __restrict__ voff,
+                                       const T* __restrict__ v, const uint8_t* __restrict__ on, Epi epi, T* sh, T* red) {
     const int inst = blk[3 * b], r0 = blk[3 * b + 1], r1 = blk[3 * b + 2];
     if (!on[inst]) return;  // the whole workgroup leaves together
     const int* P = ptr + pbase[inst];
@@ -145,6 +141,153 @@ __global__ __launch_bounds__(WG) void k_bblocks(int nblk, const int* __restrict_
     }
     for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (row < nr && lane == 0) epi(r0 + row, s);
+}
+
+// One launch over a block table (or, with a list, over the live instances' blocks: nblk = list length, so a launch
+// covers only the instances still iterating). TAG only names a second instantiation (the all-live probe launches)
+// for kernel traces.
+template <class T, int CAP, bool SORTED, class Epi, int WG = stage_wg(CAP), int TAG = 0>
+__global__ __launch_bounds__(WG) void k_bblocks(int nblk, const int* __restrict__ list, const int* __restrict__ blk,
+                                                 const int* __restrict__ pbase, const int* __restrict__ ptr,
+                                                 const int* __restrict__ idx, const T* __restrict__ val,
+                                                 const int* __restrict__ voff, const T* __restrict__ v,
+                                                 const uint8_t* __restrict__ on, Epi epi) {
+    __shared__ T sh[CAP];
+    __shared__ T red[WG / 64];
+    const int lb = xcd_block(int(blockIdx.x), nblk);
+    const int b = list ? list[lb] : lb;  // list: the live instances' blocks, in instance order
+    bblock<T, CAP, SORTED, WG>(b, blk, pbase, ptr, idx, val, voff, v, on, epi, sh, red);
+}
+
+// ------------------------------------------------------------------ persistent blocks (small live sets)
+// When few instances are live, one iteration is two launches of a few workgroups each, and each launch is bound by
+// its chain of dependent loads and its launch gap rather than by bytes (the one-LP tail of a batch: about 7 us per
+// launch). k_bpersist runs a whole block of iterations in one cooperative launch instead: workgroup g takes the
+// live row blocks g, g + G, ... and the live column blocks likewise, and every step is row phase, grid barrier,
+// column phase, grid barrier, with the same block code as k_bblocks (so results are bit-identical to the lockstep
+// launches). Before its first phase each workgroup copies the vector entries its blocks will write (z of its row
+// blocks; x, x+, w of its column blocks) to snapshot buffers, so a block that does not finish can be undone.
+//
+// The grid barrier counts arrivals on one counter (reset to 0 before each launch; arrival k of a workgroup waits
+// for k * G). Its wait is bounded: after `spins` polls, or when another workgroup has already given up, the
+// workgroup sets the error flag and returns, so every wave of the launch exits; the host then restores the
+// snapshot and runs the block with the lockstep launches.
+struct GridBar {
+    unsigned* count;
+    int* err;
+    int spins;
+    unsigned extra;  // added to every target; > 0 only to force the time-out path in tests (never reached)
+};
+
+__device__ __forceinline__ bool grid_barrier(const GridBar& g, unsigned target, int* flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int fail = 0;
+        __threadfence();  // release this workgroup's writes at agent scope
+        atomicAdd(g.count, 1u);
+        for (int k = 0; __hip_atomic_load(g.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target + g.extra;
+             ++k) {
+            if (k >= g.spins || __hip_atomic_load(g.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                atomicExch(g.err, 1);
+                fail = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();  // acquire the other workgroups' writes
+        *flag = fail;
+    }
+    __syncthreads();
+    return *flag == 0;
+}
+
+template <class T>
+struct PersistArgs {
+    int steps, na, nt;
+    const int *alist, *tlist;
+    const int *ablk, *apbase, *aptr, *aidx, *avoff;  // A side (row phase), gathers w
+    const T* aval;
+    const int *tblk, *tpbase, *tptr, *tidx, *tvoff;  // A^T side (column phase), gathers z
+    const T* tval;
+    const uint8_t* on;
+    BEpiHRow<T> er;
+    BEpiHCol<T> ec;  // ec.ATz is set on the block's last step only
+    T* ATz;
+    T *zs, *xs, *xps, *ws;  // snapshot of z (rows) and x, x+, w (columns) of the live blocks
+    GridBar bar;
+};
+
+template <class T, int CAP, bool SORTED, int WG = stage_wg(CAP)>
+__global__ __launch_bounds__(WG) void k_bpersist(PersistArgs<T> a) {
+    __shared__ T sh[CAP];
+    __shared__ T red[WG / 64];
+    __shared__ int flag;
+    const int g = int(blockIdx.x), G = int(gridDim.x), tid = int(threadIdx.x);
+    for (int i = g; i < a.na; i += G) {
+        const int b = a.alist[i], inst = a.ablk[3 * b], r0 = a.ablk[3 * b + 1], r1 = a.ablk[3 * b + 2];
+        if (!a.on[inst]) continue;
+        const int base = a.er.off[inst];
+        for (int r = r0 + tid; r < r1; r += WG) a.zs[base + r] = a.er.z[base + r];
+    }
+    for (int i = g; i < a.nt; i += G) {
+        const int b = a.tlist[i], inst = a.tblk[3 * b], r0 = a.tblk[3 * b + 1], r1 = a.tblk[3 * b + 2];
+        if (!a.on[inst]) continue;
+        const int base = a.ec.off[inst];
+        for (int r = r0 + tid; r < r1; r += WG) {
+            const int j = base + r;
+            a.xs[j] = a.ec.x[j], a.xps[j] = a.ec.xp[j], a.ws[j] = a.ec.w[j];
+        }
+    }
+    __syncthreads();
+    unsigned target = 0;
+    for (int s = 0; s < a.steps; ++s) {
+        BEpiHRow<T> er = a.er;
+        er.s = s;
+        for (int i = g; i < a.na; i += G) {
+            bblock<T, CAP, SORTED, WG>(a.alist[i], a.ablk, a.apbase, a.aptr, a.aidx, a.aval, a.avoff, a.ec.w, a.on,
+                                       er, sh, red);
+            __syncthreads();
+        }
+        target += unsigned(G);
+        if (!grid_barrier(a.bar, target, &flag)) return;
+        BEpiHCol<T> ec = a.ec;
+        ec.s = s;
+        ec.ATz = s + 1 == a.steps ? a.ATz : nullptr;
+        for (int i = g; i < a.nt; i += G) {
+            bblock<T, CAP, SORTED, WG>(a.tlist[i], a.tblk, a.tpbase, a.tptr, a.tidx, a.tval, a.tvoff, a.er.z, a.on,
+                                       ec, sh, red);
+            __syncthreads();
+        }
+        if (s + 1 < a.steps) {
+            target += unsigned(G);
+            if (!grid_barrier(a.bar, target, &flag)) return;
+        }
+    }
+}
+
+// Undoes a persistent block that did not finish: workgroup i < na restores z of live row block alist[i], the
+// others x, x+, w of live column block tlist[i - na].
+template <class T>
+__global__ __launch_bounds__(kWG) void k_brestore(PersistArgs<T> a) {
+    const int i = int(blockIdx.x), tid = int(threadIdx.x);
+    const bool row = i < a.na;
+    const int b = row ? a.alist[i] : a.tlist[i - a.na];
+    const int* blk = row ? a.ablk : a.tblk;
+    const int inst = blk[3 * b], r0 = blk[3 * b + 1], r1 = blk[3 * b + 2];
+    if (!a.on[inst]) return;
+    const int base = row ? a.er.off[inst] : a.ec.off[inst];
+    for (int r = r0 + tid; r < r1; r += kWG) {
+        const int j = base + r;
+        if (row) {
+            a.er.z[j] = a.zs[j];
+        } else {
+            a.ec.x[j] = a.xs[j], a.ec.xp[j] = a.xps[j], a.ec.w[j] = a.ws[j];
+        }
+    }
+}
+
+__global__ void k_reset_bar(unsigned* count, int* err) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0u, *err = 0;
 }
 
 // Reduction / elementwise table: (instance, q, nb) entries; entry q of nb covers local indices
@@ -355,6 +498,12 @@ public:
         size_t free0 = 0, free1 = 0, total = 0;
         HIP_OK(hipMemGetInfo(&free0, &total));
         name_ = device_name();
+        persist_max_ = o.persist_max;
+        if (const char* e = std::getenv("TPDHG_PERSIST_MAX")) persist_max_ = std::atoi(e);
+        if (const char* e = std::getenv("TPDHG_PERSIST_SPINS")) persist_spins_ = std::max(1, std::atoi(e));
+        // failure injection for tests: every grid barrier waits for one arrival too many and gives up after
+        // persist_spins_ polls, so the restore-and-replay path runs while every wave still exits
+        if (const char* e = std::getenv("TPDHG_PERSIST_TEST_UNREACHABLE")) persist_unreachable_ = std::atoi(e) > 0 ? 1u : 0u;
         HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         build_host(probs);
         upload_all(probs);
@@ -373,6 +522,7 @@ public:
         if (hpar_) (void)hipHostFree(hpar_);
         if (hout_) (void)hipHostFree(hout_);
         if (hlist_) (void)hipHostFree(hlist_);
+        if (herr_) (void)hipHostFree(herr_);
         if (st_) (void)hipStreamDestroy(st_);
     }
 
@@ -492,6 +642,11 @@ public:
         if (steps <= 0) return;
         upload_act(on);
         ensure_lists(on);
+        if (persistent_fits()) {
+            launch_persistent(steps);
+            return;
+        }
+        ++lockstep_blocks_;
         if (o_.use_graph && !profile_ && steps == o_.eval_freq) {
             if (!graph_) {
                 hipGraph_t g = nullptr;
@@ -568,6 +723,10 @@ public:
     }
     const std::string& name() const override { return name_; }
     size_t mem_used() const override { return mem_used_; }
+
+    int64_t persistent_blocks() const override { return persistent_blocks_; }
+    int64_t lockstep_blocks() const override { return lockstep_blocks_; }
+    int persist_max() const override { return persist_ok_ ? persist_max_ : 0; }
 
     KernelTimes probe(int iters) override {
         if (iters < 1) throw std::invalid_argument("probe needs at least one iteration");
@@ -860,6 +1019,83 @@ private:
         HIP_OK(hipStreamSynchronize(st_));
     }
 
+    // Persistent blocks: used while both live block lists fit one co-resident grid of at most persist_max_
+    // workgroups (TPDHG_PERSIST_MAX; 0 turns the path off), never under per-launch profiling.
+    template <int CAP, bool SRT>
+    const void* persist_kernel() {
+        return reinterpret_cast<const void*>(&k_bpersist<T, CAP, SRT>);
+    }
+    const void* persist_kernel_for_cap() {
+        switch (cap_) {
+            case 1024: return sorted_ ? persist_kernel<1024, true>() : persist_kernel<1024, false>();
+            case 4096: return sorted_ ? persist_kernel<4096, true>() : persist_kernel<4096, false>();
+            case 8192: return sorted_ ? persist_kernel<8192, true>() : persist_kernel<8192, false>();
+            default: return sorted_ ? persist_kernel<2048, true>() : persist_kernel<2048, false>();
+        }
+    }
+    int persist_grid_max() {  // co-resident workgroups of the persistent kernel at the current stage
+        auto it = persist_cap_.find(cap_);
+        if (it != persist_cap_.end()) return it->second;
+        int dev = 0, cus = 0, coop = 0, per_cu = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_OK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+        if (coop) HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_kernel_for_cap(), stage_wg(cap_), 0));
+        const int g = coop ? per_cu * cus : 0;
+        persist_cap_[cap_] = g;
+        return g;
+    }
+    bool persistent_fits() {
+        if (!persist_ok_ || persist_max_ <= 0 || profile_) return false;
+        const int need = std::max(nalist_, ntlist_);
+        return need > 0 && need <= persist_max_ && need <= persist_grid_max();
+    }
+    void launch_persistent(int steps) {
+        if (!zs_) {
+            zs_ = mem_.alloc<T>(size_t(std::max(mtot_, 1)));
+            xs_ = mem_.alloc<T>(size_t(std::max(ntot_, 1))), xps_ = mem_.alloc<T>(size_t(std::max(ntot_, 1)));
+            ws_ = mem_.alloc<T>(size_t(std::max(ntot_, 1)));
+            bar_ = mem_.alloc<unsigned>(1), err_ = mem_.alloc<int>(1);
+            HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&herr_), sizeof(int), hipHostMallocDefault));
+        }
+        const Side& a = A_;
+        const Side& t = AT_;
+        PersistArgs<T> pa{steps, nalist_, ntlist_, alist_, tlist_,
+                          a.blk, a.pbase, a.ptr, a.idx, a.voff, a.val,
+                          t.blk, t.pbase, t.ptr, t.idx, t.voff, t.val,
+                          dact_, row_epi(0), col_epi(0, 0, false), ATz_, zs_, xs_, xps_, ws_,
+                          GridBar{bar_, err_, persist_spins_, persist_unreachable_}};
+        k_reset_bar<<<1, 1, 0, st_>>>(bar_, err_);
+        const int G = std::max(nalist_, ntlist_);
+        void* args[] = {&pa};
+        const hipError_t e = hipLaunchCooperativeKernel(persist_kernel_for_cap(), dim3(unsigned(G)),
+                                                        dim3(unsigned(stage_wg(cap_))), args, 0, st_);
+        if (e != hipSuccess) {  // not launchable as one co-resident grid here: lockstep from now on
+            (void)hipGetLastError();
+            persist_ok_ = false;
+            std::fprintf(stderr, "batch: persistent blocks off (cooperative launch of %d workgroups: %s)\n", G,
+                         hipGetErrorString(e));
+            ++lockstep_blocks_;
+            launch_block(steps);
+            return;
+        }
+        HIP_OK(hipMemcpyAsync(herr_, err_, sizeof(int), hipMemcpyDeviceToHost, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        if (*herr_) {  // a grid barrier gave up: undo the partial block and run it in lockstep
+            persist_ok_ = false;
+            std::fprintf(stderr, "batch: a persistent block's grid barrier timed out; restored and replayed in "
+                                 "lockstep (persistent blocks off for this batch)\n");
+            k_brestore<T><<<nalist_ + ntlist_, kWG, 0, st_>>>(pa);
+            HIP_OK(hipGetLastError());
+            ++lockstep_blocks_;
+            launch_block(steps);
+            return;
+        }
+        k_bset_counters<<<(B_ + kWG - 1) / kWG, kWG, 0, st_>>>(B_, dact_, kc_, steps, 0);
+        HIP_OK(hipGetLastError());
+        ++persistent_blocks_;
+    }
+
     Seg rseg() const { return Seg{d_rtab_, d_roff_, d_m_}; }
     Seg cseg() const { return Seg{d_ctab_, d_coff_, d_n_}; }
 
@@ -987,6 +1223,14 @@ private:
 
     int B_;
     BatchOptions o_;
+    bool persist_ok_ = true;
+    int persist_max_ = 256, persist_spins_ = 1 << 20;
+    unsigned persist_unreachable_ = 0;
+    std::map<int, int> persist_cap_;
+    int64_t persistent_blocks_ = 0, lockstep_blocks_ = 0;
+    T *zs_ = nullptr, *xs_ = nullptr, *xps_ = nullptr, *ws_ = nullptr;
+    unsigned* bar_ = nullptr;
+    int *err_ = nullptr, *herr_ = nullptr;
     bool profile_ = false;
     Mem mem_;
     hipStream_t st_ = nullptr;

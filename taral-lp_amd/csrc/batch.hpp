@@ -57,6 +57,11 @@ public:
     // code), so a kernel trace separates these all-live launches from a solve's shrinking ones. Leaves the
     // iterates changed: the next solve starts over from its base points.
     virtual KernelTimes probe(int iters) = 0;
+    // Iteration blocks run so far as one persistent launch / as lockstep launches, and the live-block limit of the
+    // persistent path (0 when it is off).
+    virtual int64_t persistent_blocks() const = 0;
+    virtual int64_t lockstep_blocks() const = 0;
+    virtual int persist_max() const = 0;
 };
 
 struct BatchOptions {
@@ -65,6 +70,10 @@ struct BatchOptions {
     bool use_graph = true;
     double rho = 1.9;
     int cap = 0;  // nonzeros staged per row block: 1024, 2048, 4096 or 8192; 0 = pick by timing (autotune)
+    // Largest live block count (per side) run as one persistent launch per iteration block; 0 = lockstep only
+    // (the default: measured slower on the small live sets it serves, see the design notes; TPDHG_PERSIST_MAX
+    // overrides).
+    int persist_max = 0;
 };
 
 // Instances are the scaled problems; instances whose scaled A (and so A^T) is identical share the device copy.

@@ -528,6 +528,8 @@ int tpdhg_batch_get_info(const tpdhg_batch* b, tpdhg_batch_info* info) {
     info->device_mem_bytes = int64_t(d.mem_used()), info->setup_s = b->s->setup_s();
     const std::vector<float> t = d.tune_ms();
     for (size_t k = 0; k < t.size() && k < 4; ++k) info->tune_ms[k] = t[k];
+    info->persist_max = d.persist_max();
+    info->persistent_blocks = d.persistent_blocks(), info->lockstep_blocks = d.lockstep_blocks();
     return TPDHG_OK;
 }
 

@@ -24,7 +24,8 @@ def layout(request, monkeypatch):
     monkeypatch.setenv("TPDHG_LAYOUT", request.param)
     return request.param
 
-FAST = dict(tol=1e-6, precision_bits=32, halpern=1, refine=1)
+# a solve that fails to converge ends at the time limit (and reports progress) instead of running for an hour
+FAST = dict(tol=1e-6, precision_bits=32, halpern=1, refine=1, time_limit=120, progress_interval=10)
 
 
 def _mixed():
@@ -121,3 +122,60 @@ def test_batch_iteration_limit_stops_every_instance():
     rb = ffi.Batch([ffi.Model.from_lp(md) for md in lps], cap=2048, max_iter=128, **FAST).solve()
     for b in rb:
         assert b["status"] == "iteration_limit" and b["iterations"] == 128
+
+
+def _batch_run(lps, monkeypatch, cap=2048, **env):
+    for k, v in env.items():
+        monkeypatch.setenv(k, str(v))
+    b = ffi.Batch([ffi.Model.from_lp(md) for md in lps], cap=cap, **FAST)
+    res, info = b.solve(), b.info()
+    b.close()
+    for k in env:
+        monkeypatch.delenv(k)
+    return res, info
+
+
+def _same(ra, rb):
+    for a, b in zip(ra, rb):
+        assert (a["status"], a["iterations"], a["restarts"], a["refinements"]) == \
+               (b["status"], b["iterations"], b["restarts"], b["refinements"])
+        assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+
+
+def test_persistent_blocks_match_lockstep(monkeypatch):
+    """Small live sets run each 64-iteration block as one cooperative launch (k_bpersist: row phase, grid barrier,
+    column phase, grid barrier, per step) with the lockstep launches' block code: bit-identical results."""
+    lps = _mixed()
+    lock, il = _batch_run(lps, monkeypatch, TPDHG_PERSIST_MAX=0)
+    pers, ip = _batch_run(lps, monkeypatch, TPDHG_PERSIST_MAX=256)
+    print(f"lockstep blocks {il['lockstep_blocks']}; persistent {ip['persistent_blocks']} / lockstep "
+          f"{ip['lockstep_blocks']}", flush=True)
+    assert il["persistent_blocks"] == 0 and il["persist_max"] == 0
+    assert ip["persistent_blocks"] > 0 and ip["persist_max"] > 0
+    _same(lock, pers)
+
+
+def test_persistent_switches_in_when_the_live_set_shrinks(monkeypatch):
+    """Uneven LPs at a fixed stage (the autotuner times stages, so two batches may pick different ones and sum in
+    different orders): lockstep while many blocks are live, persistent once few are."""
+    lps = gen.uneven_batch(count=24, m_hi=6000)
+    lock, il = _batch_run(lps, monkeypatch, cap=1024, TPDHG_PERSIST_MAX=0)
+    pers, ip = _batch_run(lps, monkeypatch, cap=1024, TPDHG_PERSIST_MAX=160)
+    assert il["cap"] == ip["cap"] == 1024
+    print(f"persistent {ip['persistent_blocks']} / lockstep {ip['lockstep_blocks']} blocks", flush=True)
+    assert ip["persistent_blocks"] > 0 and ip["lockstep_blocks"] > 0
+    _same(lock, pers)
+    for md, r in zip(lps, pers):
+        assert max(rel_kkt(md, r["x"], r["y"])) <= 1.0001e-6
+
+
+def test_persistent_barrier_time_out_restores_and_replays(monkeypatch):
+    """Failure path forced from the host: every grid barrier waits for one arrival too many and gives up after 64
+    polls, so each workgroup sets the error flag and exits; the host restores the block's snapshot, replays it with
+    the lockstep launches and turns the persistent path off. Results equal the lockstep solve's bit for bit."""
+    lps = _mixed()
+    lock, _ = _batch_run(lps, monkeypatch, TPDHG_PERSIST_MAX=0)
+    fb, info = _batch_run(lps, monkeypatch, TPDHG_PERSIST_MAX=256, TPDHG_PERSIST_TEST_UNREACHABLE=1,
+                          TPDHG_PERSIST_SPINS=64)
+    assert info["persist_max"] == 0 and info["persistent_blocks"] == 0 and info["lockstep_blocks"] > 0
+    _same(lock, fb)
