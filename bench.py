@@ -348,6 +348,7 @@ def run_batch(a, ffi, gen, desc, lps, budget):
     res = runs[-1]
     value = ok / wall
     its = np.array([r["iterations"] for r in res])
+    info_after = batch.info()
     # roofline: one batched iteration with every instance live, both launches timed apart
     t = batch.probe(a.probe_iters)
     b_row = b_col = 0
@@ -368,7 +369,7 @@ def run_batch(a, ffi, gen, desc, lps, budget):
                        "slowest": lps[int(its.argmax())].name},
         "max_rel_kkt": max(rel_kkt_of(r) for r in res), "max_objective_rel_error_vs_planted": max(errs),
         "session_setup_s": round(setup, 3), "power_iteration": "included in every timed solve",
-        "device": res[0]["device"], "roofline": roofline})
+        "stage_switches": info_after["stage_switches"], "device": res[0]["device"], "roofline": roofline})
     if fp64:
         done = [sample[j] for j in range(len(fp64))]
         tot = sum(dt for _, _, dt in fp64)

@@ -259,9 +259,11 @@ typedef struct {
     int32_t persist_max;      /* live row/column blocks up to which an iteration block runs as one persistent
                                  launch (0: off, or switched off after a failed launch) */
     int64_t persistent_blocks, lockstep_blocks; /* iteration blocks run each way so far */
+    int64_t stage_switches;   /* row-block stage changes so far: with an autotuned stage (cap 0) the batch moves to
+                                 smaller stages as its live set shrinks below 256 blocks of the tuned stage */
 } tpdhg_batch_info;
-/* opt->halpern must be 1. cap: 0 = pick the row-block stage by timing, else 1024, 2048, 4096 or 8192. The batch keeps
- * copies of the models. */
+/* opt->halpern must be 1. cap: 0 = pick the row-block stage by timing (and follow the live set to smaller stages in
+ * the batch's tail), else 1024, 2048, 4096 or 8192 throughout. The batch keeps copies of the models. */
 int tpdhg_batch_create(const tpdhg_model* const* models, int32_t count, const tpdhg_options* opt, int32_t cap,
                        tpdhg_batch** out);
 int tpdhg_batch_get_info(const tpdhg_batch* b, tpdhg_batch_info* info);

@@ -34,13 +34,27 @@ __device__ __forceinline__ T batch_lambda(int k) {
     return T(double(k + 1) / double(k + 2));
 }
 
+// One row block of a batch: instance, local rows [r0, r1), absolute nonzero range [k0, k1) in the pool, the
+// instance's first row pointer in the pool, the offset of its segment of the gathered vector and of the vector the
+// epilogue writes. The tables and the live lists hold these records, so a workgroup finds everything about its
+// block in one 32-byte read.
+struct alignas(16) BMeta {
+    int inst, r0, r1, k0, k1, pbase, voff, off;
+};
+
+// Epilogues: load(instance, offset) reads the per-instance scalars; pre(r) loads the operands of local row r and
+// apply(r, sum, pre) finishes it (loaded early, before the gathers, so their latency overlaps them);
+// operator()(r, sum) does both at once.
+
 // y[off_i + r] = s
 template <class T>
 struct BStore {
     T* y;
-    const int* off;
     int base;
-    __device__ void load(int i) { base = off[i]; }
+    struct Pre {};
+    __device__ void load(int, int off) { base = off; }
+    __device__ Pre pre(int) const { return {}; }
+    __device__ void apply(int r, T s, const Pre&) const { y[base + r] = s; }
     __device__ void operator()(int r, T s) const { y[base + r] = s; }
 };
 
@@ -49,20 +63,25 @@ template <class T>
 struct BEpiHRow {
     T* z;
     const T *z0, *zh, *rl, *ru, *par;
-    const int *kc, *off;
+    const int* kc;
     int s;
     T rho;
     int base;
     T sig, lam;
-    __device__ void load(int i) { base = off[i], sig = par[2 * i + 1], lam = batch_lambda<T>(kc[i] + s); }
-    __device__ void operator()(int r, T a) const {
+    struct Pre {
+        T zi, h, lo, hi, z0;
+    };
+    __device__ void load(int i, int off) { base = off, sig = par[2 * i + 1], lam = batch_lambda<T>(kc[i] + s); }
+    __device__ Pre pre(int r) const {
         const int g = base + r;
-        const T zi = z[g], h = zh[g];
-        const T v = zi + sig * a, q = (v + h) / sig;
-        const T lo = rl[g], hi = ru[g];
-        const T zp = q < lo ? v - sig * lo : (q > hi ? v - sig * hi : -h);
-        z[g] = lam * (rho * zp - (rho - 1) * zi) + (1 - lam) * z0[g];
+        return Pre{z[g], zh[g], rl[g], ru[g], z0[g]};
     }
+    __device__ void apply(int r, T a, const Pre& p) const {
+        const T v = p.zi + sig * a, q = (v + p.h) / sig;
+        const T zp = q < p.lo ? v - sig * p.lo : (q > p.hi ? v - sig * p.hi : -p.h);
+        z[base + r] = lam * (rho * zp - (rho - 1) * p.zi) + (1 - lam) * p.z0;
+    }
+    __device__ void operator()(int r, T a) const { apply(r, a, pre(r)); }
 };
 
 // Primal Halpern blend (mode 0) or none (mode 1), next primal step and gather vector (fast.hip EpiHCol).
@@ -70,44 +89,52 @@ template <class T>
 struct BEpiHCol {
     T *x, *xp, *w, *ATz;
     const T *x0, *c, *l, *u, *par;
-    const int *kc, *off;
+    const int* kc;
     int s, mode;
     T rho;
     int base;
     T tau, lam;
-    __device__ void load(int i) { base = off[i], tau = par[2 * i], lam = batch_lambda<T>(kc[i] + s); }
-    __device__ void operator()(int r, T a) const {
+    struct Pre {
+        T xj, xpj, x0j, c, l, u;
+    };
+    __device__ void load(int i, int off) { base = off, tau = par[2 * i], lam = batch_lambda<T>(kc[i] + s); }
+    __device__ Pre pre(int r) const {
         const int j = base + r;
-        T xj = x[j];
+        if (mode == 0) return Pre{x[j], xp[j], x0[j], c[j], l[j], u[j]};
+        return Pre{x[j], T(0), T(0), c[j], l[j], u[j]};
+    }
+    __device__ void apply(int r, T a, const Pre& p) const {
+        const int j = base + r;
+        T xj = p.xj;
         if (mode == 0) {
-            xj = lam * (rho * xp[j] - (rho - 1) * xj) + (1 - lam) * x0[j];
+            xj = lam * (rho * p.xpj - (rho - 1) * xj) + (1 - lam) * p.x0j;
             x[j] = xj;
         }
-        const T xn = clampv(xj - tau * (c[j] + a), l[j], u[j]);
+        const T xn = clampv(xj - tau * (p.c + a), p.l, p.u);
         xp[j] = xn, w[j] = 2 * xn - xj;
         if (ATz) ATz[j] = a;
     }
+    __device__ void operator()(int r, T a) const { apply(r, a, pre(r)); }
 };
 
-// Row-block SpMV over the batch with a CAP-entry LDS stage. blk holds (instance, first row, end row) triples;
+// Row-block SpMV over the batch with a CAP-entry LDS stage, one workgroup of WG threads per block (record m);
 // rows are local to the instance. SORTED: the sorted layout of hipcommon.hpp (k_sorted) per instance: within a
-// block the nonzeros are in column order with their row-order position packed above the column; otherwise the
-// k_rowblocks scheme. A row longer than CAP is a block of its own, summed strided by the workgroup. One workgroup
-// of WG threads runs block b; sh (CAP values) and red (WG / 64) are its LDS. The whole workgroup takes the same
-// path, so the barriers inside are uniform.
+// block the nonzeros are in column order with their row-order position packed above the column, each row summed
+// in CSR order (block_row_sums' assignment of rows to threads and its summation order); the row pointers and the
+// epilogue operands of a thread's first rows are loaded before the gathers. Otherwise the k_rowblocks scheme. A
+// row longer than CAP is a block of its own, summed strided by the workgroup. sh (CAP values) and red (WG / 64)
+// are the workgroup's LDS; the whole workgroup takes the same path, so the barriers inside are uniform.
 template <class T, int CAP, bool SORTED, int WG, class Epi>
-__device__ __forceinline__ void bblock(int b, const int* __restrict__ blk, const int* __restrict__ pbase,
-                                       const int* __restrict__ ptr, const int* __restrict__ idx,
-                                       const T* __restrict__ val, const int* __restrict__ voff,
-                                       const T* __restrict__ v, const uint8_t* __restrict__ on, Epi epi, T* sh, T* red) {
-    const int inst = blk[3 * b], r0 = blk[3 * b + 1], r1 = blk[3 * b + 2];
-    if (!on[inst]) return;  // the whole workgroup leaves together
-    const int* P = ptr + pbase[inst];
-    const T* vi = v + voff[inst];
+__device__ __forceinline__ void bblock(const BMeta& m, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                       const T* __restrict__ val, const T* __restrict__ v,
+                                       const uint8_t* __restrict__ on, Epi epi, T* sh, T* red) {
+    if (!on[m.inst]) return;  // the whole workgroup leaves together
+    const int* P = ptr + m.pbase;
+    const T* vi = v + m.voff;
     const int tid = int(threadIdx.x);
-    const int k0 = P[r0], k1 = P[r1];
+    const int r0 = m.r0, r1 = m.r1, k0 = m.k0, k1 = m.k1;
     constexpr uint32_t kMask = SORTED ? (1u << kBatchColBits) - 1u : 0xffffffffu;
-    epi.load(inst);
+    epi.load(m.inst, m.off);
     if (k1 - k0 > CAP) {
         T s = 0;
         gather_rounds<WG, 8>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
@@ -118,18 +145,57 @@ __device__ __forceinline__ void bblock(int b, const int* __restrict__ blk, const
         if (tid == 0) epi(r0, tree_sum<WG / 64>(red));
         return;
     }
-    gather_rounds<WG, CAP / WG>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
-                                [&](int k, int c, T x) {
-                                    if constexpr (SORTED)
-                                        sh[uint32_t(c) >> kBatchColBits] = x;
-                                    else
-                                        sh[k - k0] = x;
-                                });
-    __syncthreads();
-    if constexpr (SORTED) {  // as hipcommon.hpp's k_sorted
-        block_row_sums<T, WG>(P, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });
+    if constexpr (SORTED) {
+        const int nr = r1 - r0;
+        constexpr int Q = 4;  // rows per thread loaded ahead (one per lane group when nr <= WG)
+        int pa[Q], pe[Q];
+        typename Epi::Pre pp[Q];
+        const bool grouped = nr <= WG;
+        int lg = 6;
+        while (grouped && lg > 0 && (nr << lg) > WG) --lg;
+        const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
+        if (grouped) {
+            if (row < nr) {
+                pa[0] = P[r0 + row] - k0, pe[0] = P[r0 + row + 1] - k0;
+                if (lane == 0) pp[0] = epi.pre(r0 + row);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int r = r0 + q * WG + tid;
+                if (r < r1) pa[q] = P[r] - k0, pe[q] = P[r + 1] - k0, pp[q] = epi.pre(r);
+            }
+        }
+        gather_rounds<WG, 8>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
+                             [&](int, int c, T x) { sh[uint32_t(c) >> kBatchColBits] = x; });
+        __syncthreads();
+        if (grouped) {
+            T s = 0;
+            if (row < nr)
+                for (int k = pa[0] + lane; k < pe[0]; k += lpr) s += sh[k];
+            for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            if (row < nr && lane == 0) epi.apply(r0 + row, s, pp[0]);
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = r0 + q * WG + tid;
+            if (r < r1) {
+                T s = 0;
+                for (int j = pa[q]; j < pe[q]; ++j) s += sh[j];
+                epi.apply(r, s, pp[q]);
+            }
+        }
+        for (int r = r0 + Q * WG + tid; r < r1; r += WG) {  // rows beyond the first Q per thread
+            T s = 0;
+            for (int j = P[r] - k0; j < P[r + 1] - k0; ++j) s += sh[j];
+            epi(r, s);
+        }
         return;
     }
+    gather_rounds<WG, CAP / WG>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
+                                [&](int k, int, T x) { sh[k - k0] = x; });
+    __syncthreads();
     const int nr = r1 - r0;
     int lg = 6;
     while (lg > 0 && (nr << lg) > WG) --lg;
@@ -143,20 +209,17 @@ __device__ __forceinline__ void bblock(int b, const int* __restrict__ blk, const
     if (row < nr && lane == 0) epi(r0 + row, s);
 }
 
-// One launch over a block table (or, with a list, over the live instances' blocks: nblk = list length, so a launch
-// covers only the instances still iterating). TAG only names a second instantiation (the all-live probe launches)
-// for kernel traces.
+// One launch over block records: a side's whole table, or a live list (the live instances' blocks, so a launch
+// covers only the instances still iterating). TAG only names a second instantiation (the all-live probe
+// launches) for kernel traces.
 template <class T, int CAP, bool SORTED, class Epi, int WG = stage_wg(CAP), int TAG = 0>
-__global__ __launch_bounds__(WG) void k_bblocks(int nblk, const int* __restrict__ list, const int* __restrict__ blk,
-                                                 const int* __restrict__ pbase, const int* __restrict__ ptr,
+__global__ __launch_bounds__(WG) void k_bblocks(int nblk, const BMeta* __restrict__ meta, const int* __restrict__ ptr,
                                                  const int* __restrict__ idx, const T* __restrict__ val,
-                                                 const int* __restrict__ voff, const T* __restrict__ v,
-                                                 const uint8_t* __restrict__ on, Epi epi) {
+                                                 const T* __restrict__ v, const uint8_t* __restrict__ on, Epi epi) {
     __shared__ T sh[CAP];
     __shared__ T red[WG / 64];
-    const int lb = xcd_block(int(blockIdx.x), nblk);
-    const int b = list ? list[lb] : lb;  // list: the live instances' blocks, in instance order
-    bblock<T, CAP, SORTED, WG>(b, blk, pbase, ptr, idx, val, voff, v, on, epi, sh, red);
+    const BMeta m = meta[xcd_block(int(blockIdx.x), nblk)];
+    bblock<T, CAP, SORTED, WG>(m, ptr, idx, val, v, on, epi, sh, red);
 }
 
 // ------------------------------------------------------------------ persistent blocks (small live sets)
@@ -204,11 +267,9 @@ __device__ __forceinline__ bool grid_barrier(const GridBar& g, unsigned target, 
 template <class T>
 struct PersistArgs {
     int steps, na, nt;
-    const int *alist, *tlist;
-    const int *ablk, *apbase, *aptr, *aidx, *avoff;  // A side (row phase), gathers w
-    const T* aval;
-    const int *tblk, *tpbase, *tptr, *tidx, *tvoff;  // A^T side (column phase), gathers z
-    const T* tval;
+    const BMeta *alist, *tlist;      // live blocks of A (row phase, gathers w) and of A^T (column phase, gathers z)
+    const int *aptr, *aidx, *tptr, *tidx;
+    const T *aval, *tval;
     const uint8_t* on;
     BEpiHRow<T> er;
     BEpiHCol<T> ec;  // ec.ATz is set on the block's last step only
@@ -224,16 +285,16 @@ __global__ __launch_bounds__(WG) void k_bpersist(PersistArgs<T> a) {
     __shared__ int flag;
     const int g = int(blockIdx.x), G = int(gridDim.x), tid = int(threadIdx.x);
     for (int i = g; i < a.na; i += G) {
-        const int b = a.alist[i], inst = a.ablk[3 * b], r0 = a.ablk[3 * b + 1], r1 = a.ablk[3 * b + 2];
-        if (!a.on[inst]) continue;
-        const int base = a.er.off[inst];
-        for (int r = r0 + tid; r < r1; r += WG) a.zs[base + r] = a.er.z[base + r];
+        const BMeta m = a.alist[i];
+        if (!a.on[m.inst]) continue;
+        const int base = m.off;
+        for (int r = m.r0 + tid; r < m.r1; r += WG) a.zs[base + r] = a.er.z[base + r];
     }
     for (int i = g; i < a.nt; i += G) {
-        const int b = a.tlist[i], inst = a.tblk[3 * b], r0 = a.tblk[3 * b + 1], r1 = a.tblk[3 * b + 2];
-        if (!a.on[inst]) continue;
-        const int base = a.ec.off[inst];
-        for (int r = r0 + tid; r < r1; r += WG) {
+        const BMeta m = a.tlist[i];
+        if (!a.on[m.inst]) continue;
+        const int base = m.off;
+        for (int r = m.r0 + tid; r < m.r1; r += WG) {
             const int j = base + r;
             a.xs[j] = a.ec.x[j], a.xps[j] = a.ec.xp[j], a.ws[j] = a.ec.w[j];
         }
@@ -244,8 +305,7 @@ __global__ __launch_bounds__(WG) void k_bpersist(PersistArgs<T> a) {
         BEpiHRow<T> er = a.er;
         er.s = s;
         for (int i = g; i < a.na; i += G) {
-            bblock<T, CAP, SORTED, WG>(a.alist[i], a.ablk, a.apbase, a.aptr, a.aidx, a.aval, a.avoff, a.ec.w, a.on,
-                                       er, sh, red);
+            bblock<T, CAP, SORTED, WG>(a.alist[i], a.aptr, a.aidx, a.aval, a.ec.w, a.on, er, sh, red);
             __syncthreads();
         }
         target += unsigned(G);
@@ -254,8 +314,7 @@ __global__ __launch_bounds__(WG) void k_bpersist(PersistArgs<T> a) {
         ec.s = s;
         ec.ATz = s + 1 == a.steps ? a.ATz : nullptr;
         for (int i = g; i < a.nt; i += G) {
-            bblock<T, CAP, SORTED, WG>(a.tlist[i], a.tblk, a.tpbase, a.tptr, a.tidx, a.tval, a.tvoff, a.er.z, a.on,
-                                       ec, sh, red);
+            bblock<T, CAP, SORTED, WG>(a.tlist[i], a.tptr, a.tidx, a.tval, a.er.z, a.on, ec, sh, red);
             __syncthreads();
         }
         if (s + 1 < a.steps) {
@@ -271,12 +330,10 @@ template <class T>
 __global__ __launch_bounds__(kWG) void k_brestore(PersistArgs<T> a) {
     const int i = int(blockIdx.x), tid = int(threadIdx.x);
     const bool row = i < a.na;
-    const int b = row ? a.alist[i] : a.tlist[i - a.na];
-    const int* blk = row ? a.ablk : a.tblk;
-    const int inst = blk[3 * b], r0 = blk[3 * b + 1], r1 = blk[3 * b + 2];
-    if (!a.on[inst]) return;
-    const int base = row ? a.er.off[inst] : a.ec.off[inst];
-    for (int r = r0 + tid; r < r1; r += kWG) {
+    const BMeta m = row ? a.alist[i] : a.tlist[i - a.na];
+    if (!a.on[m.inst]) return;
+    const int base = m.off;
+    for (int r = m.r0 + tid; r < m.r1; r += kWG) {
         const int j = base + r;
         if (row) {
             a.er.z[j] = a.zs[j];
@@ -507,13 +564,16 @@ public:
         HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         build_host(probs);
         upload_all(probs);
-        HIP_OK(hipMemGetInfo(&free1, &total));
-        mem_used_ = free0 > free1 ? free0 - free1 : 0;
         if (o.cap) {
             set_cap(o.cap);
+            tuned_cap_ = o.cap;
         } else {
             autotune();
         }
+        HIP_OK(hipMemGetInfo(&free1, &total));
+        mem_used_ = free0 > free1 ? free0 - free1 : 0;
+        // the stage follows the live set only after tuning (TPDHG_FIXED_STAGE keeps the tuned one throughout)
+        auto_stage_ = o.cap == 0 && std::getenv("TPDHG_FIXED_STAGE") == nullptr;
     }
     ~BatchDev() override {
         if (graph_) (void)hipGraphExecDestroy(graph_);
@@ -535,7 +595,14 @@ public:
     int cap() const override { return cap_; }
     std::vector<float> tune_ms() const override { return tune_ms_; }
 
+    // A solve starts with every instance live: on the tuned stage (the previous solve may have ended on a smaller
+    // one), so every solve of a batch sums in the same order from its first product on.
+    void tuned_stage() {
+        if (auto_stage_ && cap_ != tuned_cap_) set_cap(tuned_cap_);
+    }
+
     std::vector<double> power_norm(int iters) override {
+        tuned_stage();
         std::vector<double> est(static_cast<size_t>(B_), 0.0), prev(static_cast<size_t>(B_), 0.0), fac(static_cast<size_t>(B_), 1.0);
         {
             std::vector<T> hv(static_cast<size_t>(ntot_));
@@ -556,8 +623,8 @@ public:
         T *pv = pv_, *pu = pu_;
         for (int it = 0; it < iters; ++it) {
             upload_flags(on);
-            spmv_rows(A_, pv, BStore<T>{pw_, d_roff_, 0}, dflags_);
-            spmv_rows(AT_, pw_, BStore<T>{pu, d_coff_, 0}, dflags_);
+            spmv_rows(A_, pv, BStore<T>{pw_, 0}, dflags_);
+            spmv_rows(AT_, pw_, BStore<T>{pu, 0}, dflags_);
             k_bsumsq<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, pu, part_);
             k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst_, d_ccount_, dflags_, 1, part_, 1, 0, out_);
             HIP_OK(hipMemcpyAsync(hout_, out_, size_t(B_) * sizeof(double), hipMemcpyDeviceToHost, st_));
@@ -591,6 +658,7 @@ public:
     }
 
     void set_base(const std::vector<const double*>& xb, const std::vector<const double*>& zb) override {
+        tuned_stage();
         std::vector<double> hx(static_cast<size_t>(ntot_), 0.0), hz(static_cast<size_t>(mtot_), 0.0);
         for (int i = 0; i < B_; ++i) {
             if (size_t(i) < xb.size() && xb[size_t(i)])
@@ -605,8 +673,8 @@ public:
 
     void begin_round(const std::vector<uint8_t>& on) override {
         upload_flags(on);
-        spmv_rows(A64_, xb_, BStore<double>{Axb_, d_roff_, 0}, dflags_);
-        spmv_rows(AT64_, zb_, BStore<double>{ATzb_, d_coff_, 0}, dflags_);
+        spmv_flagged(A64_, xb_, BStore<double>{Axb_, 0}, on, true);
+        spmv_flagged(AT64_, zb_, BStore<double>{ATzb_, 0}, on, false);
         k_bbegin_rows<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, rl64_, ru64_, Axb_, zb_, rl_in_, ru_in_, zh_, z_, z0_);
         k_bbegin_cols<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, c64_, l64_, u64_, ATzb_, xb_, c_in_, l_in_, u_in_,
                                                   x_, x0_);
@@ -625,7 +693,7 @@ public:
         k_bcopy<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, x0_, x_);
         k_bcopy<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, z0_, z_);
         k_bset_counters<<<(B_ + kWG - 1) / kWG, kWG, 0, st_>>>(B_, dflags_, kc_, 0, 1);
-        spmv_rows(AT_, z_, col_epi(0, 1, true), dflags_);
+        spmv_flagged(AT_, z_, col_epi(0, 1, true), on, false);
         HIP_OK(hipStreamSynchronize(st_));
         HIP_OK(hipGetLastError());
     }
@@ -671,7 +739,7 @@ public:
 
     void eval(const std::vector<uint8_t>& on, std::vector<double>& out) override {
         upload_flags(on);
-        spmv_rows(A_, x_, BStore<T>{Ax_, d_roff_, 0}, dflags_);
+        spmv_flagged(A_, x_, BStore<T>{Ax_, 0}, on, true);
         BEvalArgs<T> a{Ax_, z_, z0_, x_, ATz_, x0_, Axb_, zb_, xb_, ATzb_, rl64_, ru64_, Dr64_, c64_, l64_, u64_, Dc64_};
         k_beval_rows<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, a, part_);
         k_beval_cols<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, a, part_ + size_t(nrseg_) * 5);
@@ -727,6 +795,7 @@ public:
     int64_t persistent_blocks() const override { return persistent_blocks_; }
     int64_t lockstep_blocks() const override { return lockstep_blocks_; }
     int persist_max() const override { return persist_ok_ ? persist_max_ : 0; }
+    int64_t stage_switches() const override { return stage_switches_; }
 
     KernelTimes probe(int iters) override {
         if (iters < 1) throw std::invalid_argument("probe needs at least one iteration");
@@ -750,14 +819,16 @@ public:
     }
 
 private:
-    struct Side {  // device copy of one side's pool and the block table for the current stage size
+    struct Side {  // device copy of one side's pool and the block records for the current stage size
         int rows_total = 0, nblk = 0;
-        int *ptr = nullptr, *idx = nullptr, *pbase = nullptr, *voff = nullptr, *blk = nullptr;
+        int *ptr = nullptr, *idx = nullptr;
+        BMeta* meta = nullptr;
         T* val = nullptr;
     };
     struct Side64 {
         int nblk = 0;
-        int *ptr = nullptr, *idx = nullptr, *pbase = nullptr, *voff = nullptr, *blk = nullptr;
+        int *ptr = nullptr, *idx = nullptr;
+        BMeta* meta = nullptr;
         double* val = nullptr;
     };
 
@@ -818,6 +889,12 @@ private:
         nrseg_ = int(rfirst_.empty() ? 0 : rtab_.size() / 3), ncseg_ = int(ctab_.size() / 3);
     }
 
+    BMeta* up_meta(const std::vector<BMeta>& v) {
+        BMeta* p = mem_.alloc<BMeta>(std::max<size_t>(v.size(), 1));
+        if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(BMeta), hipMemcpyHostToDevice, st_));
+        HIP_OK(hipStreamSynchronize(st_));
+        return p;
+    }
     int* up_int(const std::vector<int32_t>& v) {
         int* p = mem_.alloc<int>(std::max<size_t>(v.size(), 1));
         if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, st_));
@@ -857,20 +934,21 @@ private:
 
     void upload_all(const std::vector<const Scaled*>& probs) {
         // matrices: pointers and indices once; values in the iterate type and in FP64 (shared when T is double)
-        std::vector<int32_t> pa(static_cast<size_t>(B_)), pt(static_cast<size_t>(B_));
-        for (int i = 0; i < B_; ++i)
-            pa[size_t(i)] = PA_.pbase[size_t(mat_of_[size_t(i)])], pt[size_t(i)] = PT_.pbase[size_t(mat_of_[size_t(i)])];
-        A_.ptr = up_int(PA_.ptr), A_.idx = up_int(PA_.idx), A_.pbase = up_int(pa), A_.voff = up_int(coff_);
-        AT_.ptr = up_int(PT_.ptr), AT_.idx = up_int(PT_.idx), AT_.pbase = up_int(pt), AT_.voff = up_int(roff_);
+        // (sorted pools get their index and value arrays per stage, build_stage)
+        A_.ptr = up_int(PA_.ptr), AT_.ptr = up_int(PT_.ptr);
         A_.rows_total = mtot_, AT_.rows_total = ntot_;
-        A_.val = up_vals<T>(PA_.val), AT_.val = up_vals<T>(PT_.val);
-        A64_.ptr = A_.ptr, A64_.idx = A_.idx, A64_.pbase = A_.pbase, A64_.voff = A_.voff;
-        AT64_.ptr = AT_.ptr, AT64_.idx = AT_.idx, AT64_.pbase = AT_.pbase, AT64_.voff = AT_.voff;
-        if constexpr (sizeof(T) == sizeof(double)) {
-            A64_.val = reinterpret_cast<double*>(A_.val), AT64_.val = reinterpret_cast<double*>(AT_.val);
-        } else {
-            A64_.val = up_vals<double>(PA_.val), AT64_.val = up_vals<double>(PT_.val);
+        A64_.ptr = A_.ptr, AT64_.ptr = AT_.ptr;
+        if (!sorted_) {
+            base_idx_a_ = up_int(PA_.idx), base_idx_t_ = up_int(PT_.idx);
+            base_val_a_ = up_vals<T>(PA_.val), base_val_t_ = up_vals<T>(PT_.val);
+            if constexpr (sizeof(T) == sizeof(double)) {
+                base_val64_a_ = reinterpret_cast<double*>(base_val_a_), base_val64_t_ = reinterpret_cast<double*>(base_val_t_);
+            } else {
+                base_val64_a_ = up_vals<double>(PA_.val), base_val64_t_ = up_vals<double>(PT_.val);
+            }
         }
+        nnz_of_.resize(size_t(B_));
+        for (int i = 0; i < B_; ++i) nnz_of_[size_t(i)] = probs[size_t(i)]->A.nnz(), nnz_all_ += nnz_of_[size_t(i)];
         d_roff_ = up_int(roff_), d_coff_ = up_int(coff_);
         d_m_ = up_int(m_), d_n_ = up_int(n_);
         d_rtab_ = up_int(rtab_), d_ctab_ = up_int(ctab_);
@@ -907,34 +985,50 @@ private:
         HIP_OK(hipStreamSynchronize(st_));
     }
 
-    // Block tables of both sides for stage size `cap` (device copies; the previous ones stay allocated).
-    // Sorted pools for stage `cap`: every block's nonzeros in column order with packed positions, values permuted
-    // the same way; uploaded over the side's index and value arrays (and the FP64 twin's values).
-    void repack(const Pool& pool, const std::vector<std::vector<int32_t>>& per, bool a_side) {
+    // One row-block stage: the block records of both sides (device and host copies), each instance's range of
+    // records, and, for sorted pools, the index and value arrays permuted for this stage's blocks (every block's
+    // nonzeros in column order with packed row positions). Every stage built stays resident, so the batch can
+    // change stage between iteration blocks without touching the host.
+    struct Stage {
+        BMeta *meta_a = nullptr, *meta_t = nullptr;
+        std::vector<BMeta> hmeta_a, hmeta_t;
+        std::vector<int32_t> afirst, acount, tfirst, tcount;
+        int *idx_a = nullptr, *idx_t = nullptr;
+        T *val_a = nullptr, *val_t = nullptr;
+        double *val64_a = nullptr, *val64_t = nullptr;
+    };
+
+    // Sorted pool of one side for a stage: indices packed with row positions and values in the iterate type and
+    // in FP64, permuted by the sort, in new device arrays.
+    void repack(const Pool& pool, const std::vector<std::vector<int32_t>>& per, int cap, int*& idx, T*& val,
+                double*& val64) {
         const size_t nnz = pool.val.size();
         std::vector<int32_t> pk(nnz);
         std::vector<int64_t> perm(nnz);
         for (size_t u = 0; u < pool.rows.size(); ++u) {
             // blocks hold local rows; pool pointers are absolute, so sort_blocks sees absolute nonzero ranges
             sort_blocks(per[u], pool.ptr.data() + pool.pbase[u], pool.idx.data(), 0, nullptr, kBatchColBits, pk, perm,
-                        cap_candidate_);
+                        cap);
         }
         std::vector<double> pv(nnz);
         parallel_for(int64_t(nnz), 1 << 16, [&](int64_t k) { pv[size_t(k)] = pool.val[size_t(perm[size_t(k)])]; });
-        Side& sd = a_side ? A_ : AT_;
-        Side64& s64 = a_side ? A64_ : AT64_;
-        if (nnz) HIP_OK(hipMemcpyAsync(sd.idx, pk.data(), nnz * sizeof(int), hipMemcpyHostToDevice, st_));
+        idx = up_int(pk);
         std::vector<T> tv(nnz);
         parallel_for(int64_t(nnz), 1 << 16, [&](int64_t k) { tv[size_t(k)] = T(pv[size_t(k)]); });
-        if (nnz) HIP_OK(hipMemcpyAsync(sd.val, tv.data(), nnz * sizeof(T), hipMemcpyHostToDevice, st_));
-        if constexpr (sizeof(T) != sizeof(double))
-            if (nnz) HIP_OK(hipMemcpyAsync(s64.val, pv.data(), nnz * sizeof(double), hipMemcpyHostToDevice, st_));
+        val = mem_.alloc<T>(std::max<size_t>(nnz, 1));
+        if (nnz) HIP_OK(hipMemcpyAsync(val, tv.data(), nnz * sizeof(T), hipMemcpyHostToDevice, st_));
+        if constexpr (sizeof(T) != sizeof(double)) {
+            val64 = mem_.alloc<double>(std::max<size_t>(nnz, 1));
+            if (nnz) HIP_OK(hipMemcpyAsync(val64, pv.data(), nnz * sizeof(double), hipMemcpyHostToDevice, st_));
+        } else {
+            val64 = reinterpret_cast<double*>(val);
+        }
         HIP_OK(hipStreamSynchronize(st_));
     }
 
-    void set_cap(int cap) {
-        cap_candidate_ = cap;
-        auto table = [&](const Pool& pool) {
+    Stage& build_stage(int cap) {
+        Stage sg;
+        auto table = [&](const Pool& pool, bool a_side) {
             std::vector<std::vector<int32_t>> per(pool.rows.size());
             for (size_t u = 0; u < pool.rows.size(); ++u) {
                 // local row pointers of distinct matrix u
@@ -944,7 +1038,10 @@ private:
                 per[u] = sorted_ ? sorted_blocks(pool.rows[u], lp.data(), cap, cap)
                                  : row_blocks_cap(pool.rows[u], lp.data(), cap);
             }
-            if (sorted_) repack(pool, per, pool.val.data() == PA_.val.data());
+            if (sorted_) {
+                if (a_side) repack(pool, per, cap, sg.idx_a, sg.val_a, sg.val64_a);
+                else repack(pool, per, cap, sg.idx_t, sg.val_t, sg.val64_t);
+            }
             std::vector<int32_t> t;
             for (int i = 0; i < B_; ++i) {
                 const std::vector<int32_t>& b = per[size_t(mat_of_[size_t(i)])];
@@ -952,39 +1049,76 @@ private:
             }
             return t;
         };
-        const std::vector<int32_t> ta = table(PA_), tt = table(PT_);
-        auto ranges = [&](const std::vector<int32_t>& t, std::vector<int32_t>& first, std::vector<int32_t>& count) {
+        const std::vector<int32_t> ta = table(PA_, true), tt = table(PT_, false);
+        if (!sorted_) {  // row-block pools are not permuted: every stage reads the base arrays
+            sg.idx_a = base_idx_a_, sg.idx_t = base_idx_t_, sg.val_a = base_val_a_, sg.val_t = base_val_t_;
+            sg.val64_a = base_val64_a_, sg.val64_t = base_val64_t_;
+        }
+        // block records: A's rows gather the column-side vector and write the row side, A^T's the reverse
+        auto records = [&](const std::vector<int32_t>& t, const Pool& pool, const std::vector<int32_t>& voff,
+                           const std::vector<int32_t>& off) {
+            std::vector<BMeta> m(t.size() / 3);
+            for (size_t k = 0; k < m.size(); ++k) {
+                const int i = t[3 * k], r0 = t[3 * k + 1], r1 = t[3 * k + 2];
+                const int pb = pool.pbase[size_t(mat_of_[size_t(i)])];
+                m[k] = BMeta{i, r0, r1, pool.ptr[size_t(pb + r0)], pool.ptr[size_t(pb + r1)], pb, voff[size_t(i)],
+                             off[size_t(i)]};
+            }
+            return m;
+        };
+        sg.hmeta_a = records(ta, PA_, coff_, roff_), sg.hmeta_t = records(tt, PT_, roff_, coff_);
+        auto ranges = [&](const std::vector<BMeta>& m, std::vector<int32_t>& first, std::vector<int32_t>& count) {
             first.assign(size_t(B_), 0), count.assign(size_t(B_), 0);
-            for (size_t k = 0; k < t.size() / 3; ++k) {
-                const int i = t[3 * k];
+            for (size_t k = 0; k < m.size(); ++k) {
+                const int i = m[k].inst;
                 if (count[size_t(i)] == 0) first[size_t(i)] = int32_t(k);
                 count[size_t(i)]++;
             }
         };
-        ranges(ta, afirst_, acount_), ranges(tt, tfirst_, tcount_);
+        ranges(sg.hmeta_a, sg.afirst, sg.acount), ranges(sg.hmeta_t, sg.tfirst, sg.tcount);
+        sg.meta_a = up_meta(sg.hmeta_a), sg.meta_t = up_meta(sg.hmeta_t);
         {  // live-block list buffers, grown to the largest table seen
-            const size_t na = std::max<size_t>(ta.size() / 3, 1), nt = std::max<size_t>(tt.size() / 3, 1);
-            if (na > list_cap_a_) alist_ = mem_.alloc<int>(na), list_cap_a_ = na;
-            if (nt > list_cap_t_) tlist_ = mem_.alloc<int>(nt), list_cap_t_ = nt;
+            const size_t na = std::max<size_t>(sg.hmeta_a.size(), 1), nt = std::max<size_t>(sg.hmeta_t.size(), 1);
+            if (na > list_cap_a_) alist_ = mem_.alloc<BMeta>(na), list_cap_a_ = na;
+            if (nt > list_cap_t_) tlist_ = mem_.alloc<BMeta>(nt), list_cap_t_ = nt;
             if (na + nt > hlist_cap_) {
                 if (hlist_) HIP_OK(hipHostFree(hlist_));
                 hlist_ = nullptr;
-                HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hlist_), (na + nt) * sizeof(int), hipHostMallocDefault));
+                HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hlist_), (na + nt) * sizeof(BMeta), hipHostMallocDefault));
                 hlist_cap_ = na + nt;
             }
         }
-        list_inst_.clear(), nalist_ = ntlist_ = 0;
-        auto it = tables_.find(cap);
-        if (it == tables_.end()) {
-            it = tables_.emplace(cap, std::make_pair(up_int(ta), up_int(tt))).first;
-        }
-        A_.blk = A64_.blk = it->second.first, AT_.blk = AT64_.blk = it->second.second;
-        A_.nblk = A64_.nblk = int(ta.size() / 3), AT_.nblk = AT64_.nblk = int(tt.size() / 3);
+        return stages_.emplace(cap, std::move(sg)).first->second;
+    }
+
+    // Makes stage `cap` current (building it first if needed); the live lists and the captured graph start over.
+    void set_cap(int cap) {
+        auto it = stages_.find(cap);
+        Stage& sg = it == stages_.end() ? build_stage(cap) : it->second;
+        A_.meta = A64_.meta = sg.meta_a, AT_.meta = AT64_.meta = sg.meta_t;
+        A_.idx = A64_.idx = sg.idx_a, AT_.idx = AT64_.idx = sg.idx_t;
+        A_.val = sg.val_a, AT_.val = sg.val_t, A64_.val = sg.val64_a, AT64_.val = sg.val64_t;
+        A_.nblk = A64_.nblk = int(sg.hmeta_a.size()), AT_.nblk = AT64_.nblk = int(sg.hmeta_t.size());
+        cur_ = &sg;
         cap_ = cap;
+        list_inst_.clear(), nalist_ = ntlist_ = 0;
         if (graph_) {
             (void)hipGraphExecDestroy(graph_);
             graph_ = nullptr;
         }
+    }
+
+    // Stage for a live set of `nnz` nonzeros (per side) when the stage was autotuned: the tuned stage while all
+    // instances are live or while the live nonzeros still make kStageFill blocks of it; below that the largest
+    // smaller stage that still makes kStageFill blocks, else the smallest. A nearly idle GPU runs the few blocks of
+    // a batch's tail faster when they are smaller (one live 13k x 28k LP: 10.5 us per iteration with 1,024-nonzero
+    // blocks against 15.8 us with 8,192; eight 8k-14k-row LPs: 18.5 us with 2,048 against 20.0 us).
+    int stage_for(int64_t nnz, bool all_live) const {
+        if (!auto_stage_) return cap_;  // a given stage, or the autotuner timing each stage in turn
+        if (all_live || nnz >= int64_t(kStageFill) * tuned_cap_) return tuned_cap_;
+        for (int c = tuned_cap_ / 2; c >= 1024; c /= 2)
+            if (nnz >= int64_t(kStageFill) * c) return c;
+        return 1024;
     }
 
     // Times a few iteration blocks for each stage size and keeps the fastest (all instances active).
@@ -1015,6 +1149,7 @@ private:
         (void)hipEventDestroy(e1);
         o_.use_graph = saved_graph;
         set_cap(best_cap);
+        tuned_cap_ = best_cap;
         HIP_OK(hipMemsetAsync(kc_, 0, size_t(B_) * sizeof(int), st_));
         HIP_OK(hipStreamSynchronize(st_));
     }
@@ -1060,9 +1195,7 @@ private:
         }
         const Side& a = A_;
         const Side& t = AT_;
-        PersistArgs<T> pa{steps, nalist_, ntlist_, alist_, tlist_,
-                          a.blk, a.pbase, a.ptr, a.idx, a.voff, a.val,
-                          t.blk, t.pbase, t.ptr, t.idx, t.voff, t.val,
+        PersistArgs<T> pa{steps, nalist_, ntlist_, alist_, tlist_, a.ptr, a.idx, t.ptr, t.idx, a.val, t.val,
                           dact_, row_epi(0), col_epi(0, 0, false), ATz_, zs_, xs_, xps_, ws_,
                           GridBar{bar_, err_, persist_spins_, persist_unreachable_}};
         k_reset_bar<<<1, 1, 0, st_>>>(bar_, err_);
@@ -1111,13 +1244,13 @@ private:
     }
 
     template <class S, class U, class Epi, int TAG = 0>
-    void spmv_rows(const S& side, const U* v, const Epi& epi, const uint8_t* on, const int* list = nullptr,
+    void spmv_rows(const S& side, const U* v, const Epi& epi, const uint8_t* on, const BMeta* list = nullptr,
                    int nlist = 0) {
         const int nb = list ? nlist : side.nblk;
         if (nb == 0) return;
 #define TPDHG_BB(CAPV, SRT)                                                                                      \
     k_bblocks<U, CAPV, SRT, Epi, stage_wg(CAPV), TAG><<<nb, stage_wg(CAPV), 0, st_>>>(                          \
-        nb, list, side.blk, side.pbase, side.ptr, side.idx, side.val, side.voff, v, on, epi)
+        nb, list ? list : side.meta, side.ptr, side.idx, side.val, v, on, epi)
         if (sorted_) {
             switch (cap_) {
                 case 1024: TPDHG_BB(1024, true); break;
@@ -1135,29 +1268,53 @@ private:
         }
 #undef TPDHG_BB
     }
+    // A product over the instances flagged in `on` (dflags_ holds the same flags): over the live lists when they
+    // hold every flagged instance (a batch's tail then launches a few blocks instead of the whole table), else over
+    // the side's whole table. a_side: the list of A (else of A^T).
+    template <class S, class U, class Epi>
+    void spmv_flagged(const S& side, const U* v, const Epi& epi, const std::vector<uint8_t>& on, bool a_side) {
+        bool cover = list_inst_.size() == size_t(B_);
+        for (int i = 0; i < B_ && cover; ++i) cover = !on[size_t(i)] || list_inst_[size_t(i)];
+        if (cover) spmv_rows(side, v, epi, dflags_, a_side ? alist_ : tlist_, a_side ? nalist_ : ntlist_);
+        else spmv_rows(side, v, epi, dflags_);
+    }
+
     // Live-block lists of both sides for the instances in `on`: rebuilt when an instance in `on` is missing from
     // the current lists or when the live blocks have dropped to half the lists' length (so at most ~log2 of the
     // block count rebuilds per solve); a rebuild drops the captured graph, whose launches have the old lengths.
+    // With an autotuned stage the stage follows the live set first (stage_for), which also starts the lists over.
     void ensure_lists(const std::vector<uint8_t>& on) {
+        int64_t live_nnz = 0;
+        bool all_live = true;
+        for (int i = 0; i < B_; ++i) {
+            if (on[size_t(i)]) live_nnz += nnz_of_[size_t(i)];
+            else all_live = false;
+        }
+        const int want = stage_for(live_nnz, all_live);
+        if (want != cap_) {
+            set_cap(want);
+            ++stage_switches_;
+        }
+        const Stage& sg = *cur_;
         int64_t live_a = 0;
         bool missing = list_inst_.size() != size_t(B_);
         for (int i = 0; i < B_ && !missing; ++i) missing = on[size_t(i)] && !list_inst_[size_t(i)];
         for (int i = 0; i < B_; ++i)
-            if (on[size_t(i)]) live_a += acount_[size_t(i)];
+            if (on[size_t(i)]) live_a += sg.acount[size_t(i)];
         if (!missing && 2 * live_a > int64_t(nalist_)) return;
-        std::vector<int32_t> la, lt;
+        std::vector<BMeta> la, lt;
         list_inst_.assign(size_t(B_), 0);
         for (int i = 0; i < B_; ++i) {
             if (!on[size_t(i)]) continue;
             list_inst_[size_t(i)] = 1;
-            for (int k = 0; k < acount_[size_t(i)]; ++k) la.push_back(afirst_[size_t(i)] + k);
-            for (int k = 0; k < tcount_[size_t(i)]; ++k) lt.push_back(tfirst_[size_t(i)] + k);
+            for (int k = 0; k < sg.acount[size_t(i)]; ++k) la.push_back(sg.hmeta_a[size_t(sg.afirst[size_t(i)] + k)]);
+            for (int k = 0; k < sg.tcount[size_t(i)]; ++k) lt.push_back(sg.hmeta_t[size_t(sg.tfirst[size_t(i)] + k)]);
         }
         std::copy(la.begin(), la.end(), hlist_);
         std::copy(lt.begin(), lt.end(), hlist_ + la.size());
-        if (!la.empty()) HIP_OK(hipMemcpyAsync(alist_, hlist_, la.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+        if (!la.empty()) HIP_OK(hipMemcpyAsync(alist_, hlist_, la.size() * sizeof(BMeta), hipMemcpyHostToDevice, st_));
         if (!lt.empty())
-            HIP_OK(hipMemcpyAsync(tlist_, hlist_ + la.size(), lt.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+            HIP_OK(hipMemcpyAsync(tlist_, hlist_ + la.size(), lt.size() * sizeof(BMeta), hipMemcpyHostToDevice, st_));
         HIP_OK(hipStreamSynchronize(st_));
         nalist_ = int(la.size()), ntlist_ = int(lt.size());
         ++list_builds_;
@@ -1167,11 +1324,11 @@ private:
         }
     }
     BEpiHCol<T> col_epi(int s, int mode, bool store) {
-        return BEpiHCol<T>{x_, xp_, w_, store ? ATz_ : nullptr, x0_, c_in_, l_in_, u_in_, par_, kc_, d_coff_, s, mode,
+        return BEpiHCol<T>{x_, xp_, w_, store ? ATz_ : nullptr, x0_, c_in_, l_in_, u_in_, par_, kc_, s, mode,
                            T(o_.rho), 0, T(0), T(0)};
     }
     BEpiHRow<T> row_epi(int s) {
-        return BEpiHRow<T>{z_, z0_, zh_, rl_in_, ru_in_, par_, kc_, d_roff_, s, T(o_.rho), 0, T(0), T(0)};
+        return BEpiHRow<T>{z_, z0_, zh_, rl_in_, ru_in_, par_, kc_, s, T(o_.rho), 0, T(0), T(0)};
     }
     hipEvent_t event(size_t i) {
         while (events_.size() <= i) {
@@ -1240,15 +1397,22 @@ private:
     int64_t nnz_total_ = 0;
     Pool PA_, PT_;
     std::vector<int32_t> rtab_, ctab_, rfirst_, rcount_, cfirst_, ccount_;
-    int nrseg_ = 0, ncseg_ = 0, cap_ = kCap, cap_candidate_ = kCap;
+    int nrseg_ = 0, ncseg_ = 0, cap_ = kCap, tuned_cap_ = kCap;
+    static constexpr int kStageFill = 256;  // blocks that keep every CU busy
+    bool auto_stage_ = false;               // the stage was autotuned, so it may change with the live set
+    std::map<int, Stage> stages_;
+    Stage* cur_ = nullptr;
+    std::vector<int64_t> nnz_of_;            // nonzeros of each instance's A
+    int64_t nnz_all_ = 0;
+    int *base_idx_a_ = nullptr, *base_idx_t_ = nullptr;
+    T *base_val_a_ = nullptr, *base_val_t_ = nullptr;
+    double *base_val64_a_ = nullptr, *base_val64_t_ = nullptr;
     bool sorted_ = false;  // sorted pools (every instance below 2^19 rows and columns)
-    std::map<int, std::pair<int*, int*>> tables_;
-    std::vector<int32_t> afirst_, acount_, tfirst_, tcount_;
     std::vector<uint8_t> list_inst_;
-    int *alist_ = nullptr, *tlist_ = nullptr, *hlist_ = nullptr;
+    BMeta *alist_ = nullptr, *tlist_ = nullptr, *hlist_ = nullptr;
     size_t list_cap_a_ = 0, list_cap_t_ = 0, hlist_cap_ = 0;
     int nalist_ = 0, ntlist_ = 0;
-    int64_t list_builds_ = 0;
+    int64_t list_builds_ = 0, stage_switches_ = 0;
     std::vector<float> tune_ms_;
     Side A_, AT_;
     Side64 A64_, AT64_;

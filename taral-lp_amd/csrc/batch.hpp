@@ -62,6 +62,8 @@ public:
     virtual int64_t persistent_blocks() const = 0;
     virtual int64_t lockstep_blocks() const = 0;
     virtual int persist_max() const = 0;
+    // Stage changes made so far as the live set shrank (0 with a given stage).
+    virtual int64_t stage_switches() const = 0;
 };
 
 struct BatchOptions {

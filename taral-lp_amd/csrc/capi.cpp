@@ -530,6 +530,7 @@ int tpdhg_batch_get_info(const tpdhg_batch* b, tpdhg_batch_info* info) {
     for (size_t k = 0; k < t.size() && k < 4; ++k) info->tune_ms[k] = t[k];
     info->persist_max = d.persist_max();
     info->persistent_blocks = d.persistent_blocks(), info->lockstep_blocks = d.lockstep_blocks();
+    info->stage_switches = d.stage_switches();
     return TPDHG_OK;
 }
 

@@ -102,7 +102,7 @@ class BatchInfo(C.Structure):
     _fields_ = [("count", C.c_int32), ("unique_matrices", C.c_int32), ("cap", C.c_int32),
                 ("nnz_total", C.c_int64), ("nnz_stored", C.c_int64), ("device_mem_bytes", C.c_int64),
                 ("setup_s", C.c_double), ("tune_ms", C.c_float * 4), ("persist_max", C.c_int32),
-                ("persistent_blocks", C.c_int64), ("lockstep_blocks", C.c_int64)]
+                ("persistent_blocks", C.c_int64), ("lockstep_blocks", C.c_int64), ("stage_switches", C.c_int64)]
 
 
 _lib = None

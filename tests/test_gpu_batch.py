@@ -179,3 +179,20 @@ def test_persistent_barrier_time_out_restores_and_replays(monkeypatch):
                           TPDHG_PERSIST_SPINS=64)
     assert info["persist_max"] == 0 and info["persistent_blocks"] == 0 and info["lockstep_blocks"] > 0
     _same(lock, fb)
+
+
+def test_stage_follows_the_live_set():
+    """With an autotuned stage the batch moves to smaller row-block stages as its live set shrinks (resident
+    per-stage pools): the stage changes, every instance still meets 1e-6 by the independent KKT recheck, and a
+    second solve of the same batch (starting all live at the tuned stage again) repeats the first bit for bit."""
+    lps = gen.uneven_batch(count=16, m_hi=8000)
+    b = ffi.Batch([ffi.Model.from_lp(md) for md in lps], **FAST)
+    r1 = b.solve()
+    info = b.info()
+    r2 = b.solve()
+    b.close()
+    print(f"stage {info['cap']} switches {info['stage_switches']}", flush=True)
+    assert info["stage_switches"] > 0
+    _same(r1, r2)
+    for md, r in zip(lps, r1):
+        assert r["status"] == "near_optimal" and max(rel_kkt(md, r["x"], r["y"])) <= 1.0001e-6
