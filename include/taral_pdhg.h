@@ -176,6 +176,10 @@ typedef struct {
     int32_t reuse_norm; /* 1: a session or batch runs the power iteration on its first solve only and reuses ||A_s||
                            afterwards (A_s never changes within one); 0 (default): every solve runs it, as each
                            reference CLI call does (gpu/pdhg.cu:763) */
+    int32_t cpu_threads; /* -1 (default): the GPU. >= 0: the reference scheme on the CPU backend (the reference's
+                            --device cpu --threads N, gpu/pdhg.cu:517-741, 913-934) with this many host threads,
+                            0 = the reference's choice (1 below 200,000 nonzeros, else every hardware thread);
+                            results do not depend on the thread count. halpern and refine must be 0 */
 } tpdhg_options;
 void tpdhg_options_default(tpdhg_options* o);
 
@@ -196,6 +200,7 @@ typedef struct {
     char device[64];
     int32_t warm_start_used;  /* the solve started from the caller's x0/y0 (0 after a fallback or a cold start) */
     double warm_kkt, cold_kkt; /* relative KKT of both starts when warm_fallback compared them, else 0 */
+    int32_t threads;  /* host threads of the CPU backend; 0 for a GPU solve (the JSON's "backend" and "threads") */
 } tpdhg_result;
 
 /* Scales, uploads and solves (gpu/pdhg.cu:913-935 -> finish -> run). x (n) receives the original-space point

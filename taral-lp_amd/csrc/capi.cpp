@@ -83,6 +83,7 @@ tpdhg::Options to_options(const tpdhg_options* o) {
     r.omega0 = o->primal_weight, r.warm_fallback = o->warm_fallback != 0, r.progress_s = o->progress_interval;
     r.omega_rule = o->omega_rule;
     r.reuse_norm = o->reuse_norm != 0;
+    r.cpu_threads = o->cpu_threads;
     return r;
 }
 
@@ -367,6 +368,7 @@ void tpdhg_options_default(tpdhg_options* o) {
     o->primal_weight = d.omega0, o->warm_fallback = d.warm_fallback ? 1 : 0, o->progress_interval = d.progress_s;
     o->omega_rule = d.omega_rule;
     o->reuse_norm = d.reuse_norm ? 1 : 0;
+    o->cpu_threads = d.cpu_threads;
 }
 
 namespace {
@@ -389,6 +391,7 @@ void fill_result(const tpdhg::Report& r, const tpdhg::LP& lp, tpdhg_result* res,
     std::snprintf(res->device, sizeof res->device, "%s", r.device_name.c_str());
     res->warm_start_used = r.out.warm_used ? 1 : 0;
     res->warm_kkt = r.out.warm_kkt, res->cold_kkt = r.out.cold_kkt;
+    res->threads = r.threads;
     if (x) std::memcpy(x, r.x.data(), r.x.size() * sizeof(double));
     if (y) std::memcpy(y, r.y.data(), r.y.size() * sizeof(double));
 }
@@ -399,6 +402,9 @@ int check_options(const tpdhg::Options& o) {
     if (o.halpern && !(o.rho >= 1 && o.rho < 2)) return fail(TPDHG_ERR_ARG, "reflection must be in [1, 2)");
     if (!(o.omega0 >= 0) || !std::isfinite(o.omega0)) return fail(TPDHG_ERR_ARG, "primal_weight must be finite and >= 0");
     if (o.omega_rule < 0 || o.omega_rule > 3) return fail(TPDHG_ERR_ARG, "omega_rule must be 0, 1, 2 or 3");
+    if (o.cpu_threads < -1 || o.cpu_threads > 4096) return fail(TPDHG_ERR_ARG, "cpu_threads must be -1 (GPU) or 0..4096");
+    if (o.cpu_threads >= 0 && o.halpern)
+        return fail(TPDHG_ERR_ARG, "the CPU backend runs the reference scheme: set halpern = 0 and refine = 0");
     return TPDHG_OK;
 }
 }  // namespace
@@ -477,6 +483,7 @@ int tpdhg_result_json(const tpdhg_model* model, const tpdhg_options* opt, const 
         r.tm.transfer = res->transfer_s;
         r.device_name = res->device, r.device_mem_bytes = size_t(res->device_mem_bytes), r.nnz = res->nnz;
         r.out.warm_used = res->warm_start_used != 0;
+        r.threads = res->threads, r.backend = res->threads > 0 ? "cpu" : "gpu";
         std::string s = tpdhg::report_json(r, model->lp, to_options(opt));
         std::snprintf(buf, size_t(len), "%s", s.c_str());
         return TPDHG_OK;

@@ -1,8 +1,8 @@
 // `pdhg` command line: a drop-in for the reference CLI (gpu/pdhg.cu:937-992). Same flags, stdout status line,
 // JSON keys, .sol/.dual files and exit codes (0 after any solve, 1 on a parse error, 2 on usage errors and
-// unsupported QP/MILP input, which is refused before any device work or output file). The solver runs on the
-// GPU only: `--device cpu` is accepted by the parser (so scripts keep working) but ends with exit 2 and a
-// message, after the input gate, instead of running a CPU solver.
+// unsupported QP/MILP input, which is refused before any device work or output file). `--device gpu` runs the
+// HIP backend, `--device cpu [--threads N]` the reference scheme on host threads (the reference's CPU backend;
+// N = 0 or absent: one thread below 200,000 nonzeros, else every hardware thread).
 //
 // Additional flags: --fast (the fast path: FP32 reflected Halpern PDHG iterates inside FP64 iterative
 // refinement = --fp32 --halpern --refine), --halpern, --reflection R, --refine,
@@ -141,7 +141,7 @@ int main(int argc, char** argv) {
     const char *warm_sol = nullptr, *warm_dual = nullptr, *batch_list = nullptr;
     bool do_recheck = false;
     std::string device = "gpu";
-    int threads = 0;  // accepted for compatibility; the GPU path has no host threads to size
+    int threads = 0;  // CPU backend threads (0: the reference's automatic choice)
     Options o;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -204,11 +204,13 @@ int main(int argc, char** argv) {
         return 2;
     }
     tm.parse = since(t_start);
-    (void)threads;
     if (o.refine && !o.halpern) o.halpern = true;
-    if (device == "cpu") {
-        std::fprintf(stderr, "this solver runs on the GPU only; use --device gpu (there is no CPU backend)\n");
-        return 2;
+    if (device == "cpu") {  // the reference's CPU backend (gpu/pdhg.cu:914-934): the reference scheme on host threads
+        if (o.halpern) {
+            std::fprintf(stderr, "the fast path (--fast, --halpern, --refine) runs on the GPU only; use --device gpu\n");
+            return 2;
+        }
+        o.cpu_threads = std::max(0, threads);
     }
     std::vector<double> x0, y0;
     if (warm_sol && !read_named(warm_sol, lp.col_names, x0)) {

@@ -62,6 +62,12 @@ struct DeviceOptions {
 // std::runtime_error on any HIP failure; with no HIP device the error says so (there is no CPU path).
 std::unique_ptr<Device> make_device(const Scaled& P, const DeviceOptions& opt);
 
+// The reference scheme on host threads (the reference's CPU backend, `--device cpu`): threads > 0 threads, or 0
+// for the reference's choice (one below 200,000 nonzeros, else every hardware thread). Results do not depend on
+// the thread count.
+std::unique_ptr<Device> make_cpu_device(const Scaled& P, const DeviceOptions& opt, int threads);
+int cpu_threads_for(int64_t nnz, int threads);
+
 // Same from raw scaled arrays (A and A^T as CSR, int32 indices).
 std::unique_ptr<Device> make_device_raw(int m, int n, int64_t nnz, const int32_t* a_ptr, const int32_t* a_idx,
                                         const double* a_val, const int32_t* t_ptr, const int32_t* t_idx,

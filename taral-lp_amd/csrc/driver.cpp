@@ -273,8 +273,20 @@ Session::Session(const LP& lp, const Options& o, Timings tm) : lp_(lp), o_(o), t
                           lp.maximize ? -1.0 : 1.0);
     tm_.scale = since(t);
     t = Clock::now();
+    if (o.cpu_threads >= 0) {  // the reference's CPU backend: reference scheme only
+        if (o.halpern || o.refine)
+            throw std::invalid_argument("the fast path (Halpern iterations, refinement) runs on the GPU only");
+        DeviceOptions dopt;
+        dopt.precision_bits = o.precision_bits, dopt.eval_freq = o.eval_freq, dopt.use_graph = false;
+        threads_ = cpu_threads_for(P_.A.nnz(), o.cpu_threads);
+        dev_ = make_cpu_device(P_, dopt, threads_);
+        tm_.transfer = since(t);
+        setup_s_ = tm_.parse + tm_.scale + tm_.init + tm_.transfer;
+        return;
+    }
     if (device_count() == 0)
-        throw std::runtime_error("no HIP device visible: the PDHG solver runs on an AMD GPU only (there is no CPU path)");
+        throw std::runtime_error("no HIP device visible: the PDHG solver runs on an AMD GPU only (--device cpu runs "
+                                 "the reference scheme on host threads)");
     tm_.init = since(t);
     t = Clock::now();
     if (o.halpern) {
@@ -320,6 +332,7 @@ Report Session::solve(const double* x0_orig, const double* y0_orig) {
     rep.tm = tm_;
     rep.setup_s = setup_s_;
     rep.device_name = fast_ ? fast_->name() : dev_->name();
+    rep.backend = threads_ > 0 ? "cpu" : "gpu", rep.threads = threads_;
     rep.device_mem_bytes = fast_ ? fast_->mem_used() : dev_->mem_used();
     rep.nnz = P.A.nnz();
     std::vector<double> xs0, zs0;
@@ -650,7 +663,7 @@ std::string report_json(const Report& r, const LP& lp, const Options& o) {
         "\"warm_start_used\": %s}\n",
         q.status.c_str(), r.objective, q.rel_primal, q.rel_dual, q.rel_gap, o.tol, long(q.iterations), q.restarts,
         q.from_average ? "average" : "current", r.max_row_violation, r.max_bound_violation, r.setup_s, r.tm.parse,
-        r.tm.scale, r.tm.init, r.tm.transfer, r.solve_s, q.power_s, r.total_s, r.device_name.c_str(), "gpu", 0,
+        r.tm.scale, r.tm.init, r.tm.transfer, r.solve_s, q.power_s, r.total_s, r.device_name.c_str(), r.backend.c_str(), r.threads,
         o.precision_bits == 32 ? "fp32" : "fp64", lp.m, lp.n, long(r.nnz), q.norm_a, q.omega, o.eval_freq,
         o.graph ? "true" : "false", r.device_mem_bytes,
         (std::string(o.halpern ? "halpern-" : "") +

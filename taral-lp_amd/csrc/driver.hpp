@@ -51,6 +51,7 @@ struct Options {  // defaults of gpu/pdhg.cu:744-750 plus the options added here
     int omega_rule = 3;          // fast path and batches: primal-weight guard (driver.cpp:next_omega); the reference
                                  // scheme always uses the reference update
     bool reuse_norm = false;     // sessions and batches: power iteration on the first solve only
+    int cpu_threads = -1;        // >= 0: the reference scheme on the CPU backend with this many threads (0: auto)
 };
 
 struct Outcome {
@@ -88,7 +89,8 @@ struct Report {  // what the CLI prints and the C ABI returns (gpu/pdhg.cu:866-8
     double objective = 0, max_row_violation = 0, max_bound_violation = 0;
     double setup_s = 0, solve_s = 0, total_s = 0;
     Timings tm;
-    std::string device_name;
+    std::string device_name, backend = "gpu";
+    int threads = 0;  // host threads of the CPU backend (0 on the GPU)
     size_t device_mem_bytes = 0;
     int64_t nnz = 0;  // nonzeros of the scaled matrix (explicit zeros dropped)
     std::vector<double> x, y;  // original space: x = Dc x_s, y = Dr z_s
@@ -122,6 +124,7 @@ private:
     std::unique_ptr<FastDevice> fast_;
     double probe_kkt(const double* xs, const double* zs);  // relative KKT at a scaled-space start point
     double setup_s_ = 0, norm_a_ = 0;
+    int threads_ = 0;  // CPU backend threads (0: GPU)
     std::chrono::steady_clock::time_point created_;
 };
 

@@ -74,7 +74,7 @@ class Options(C.Structure):
                 ("use_graph", C.c_int32), ("verbose", C.c_int32), ("profile_kernels", C.c_int32),
                 ("halpern", C.c_int32), ("reflection", C.c_double), ("primal_weight", C.c_double),
                 ("warm_fallback", C.c_int32), ("progress_interval", C.c_double), ("omega_rule", C.c_int32),
-                ("reuse_norm", C.c_int32)]
+                ("reuse_norm", C.c_int32), ("cpu_threads", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -88,7 +88,8 @@ class Result(C.Structure):
                 ("primal_weight", C.c_double), ("device_mem_bytes", C.c_int64), ("m", C.c_int32), ("n", C.c_int32),
                 ("nnz", C.c_int64), ("row_kernel_ms", C.c_double), ("col_kernel_ms", C.c_double),
                 ("row_launches", C.c_int64), ("col_launches", C.c_int64), ("device", C.c_char * 64),
-                ("warm_start_used", C.c_int32), ("warm_kkt", C.c_double), ("cold_kkt", C.c_double)]
+                ("warm_start_used", C.c_int32), ("warm_kkt", C.c_double), ("cold_kkt", C.c_double),
+                ("threads", C.c_int32)]
 
 
 class RecheckResult(C.Structure):

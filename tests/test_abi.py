@@ -74,10 +74,10 @@ def test_cli_parse_error_and_usage(tmp_path):
     r = subprocess.run([ffi.CLI_PATH, os.path.join(MPS, "gate_continuous_lp.mps"), "--device", "tpu"],
                        capture_output=True, text=True)
     assert r.returncode == 2
-    # --device cpu on a supported model: refused (the solver has no CPU backend), no file written
+    # the fast path on the CPU backend: refused (it runs on the GPU only), no file written
     out = tmp_path / "x.json"
-    r = subprocess.run([ffi.CLI_PATH, os.path.join(MPS, "gate_continuous_lp.mps"), "--device", "cpu", "--json",
-                        str(out)], capture_output=True, text=True)
+    r = subprocess.run([ffi.CLI_PATH, os.path.join(MPS, "gate_continuous_lp.mps"), "--device", "cpu", "--fast",
+                        "--json", str(out)], capture_output=True, text=True)
     assert r.returncode == 2 and "GPU only" in r.stderr and not out.exists()
 
 
