@@ -69,3 +69,16 @@ def test_cli_device_cpu(transport_100x100, tmp_path):
     r = subprocess.run([ffi.CLI_PATH, str(mps), "--device", "cpu", "--tol", "1e-4", "--json", str(js)],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and json.load(open(js))["threads"] == 1
+
+
+LITERATURE = [("ref_steel_production.mps", 192000.0), ("ref_transportation.mps", 153.675)]
+
+
+@pytest.mark.parametrize("name,optimum", LITERATURE)
+def test_cpu_backend_literature_lps(golden_dir, name, optimum):
+    """examples/literature_lp/*.mps, optima from results/literature_lp_20261003/results.json (the reference's exact
+    dual simplex, HiGHS agreeing)."""
+    md = ffi.Model.from_mps(os.path.join(golden_dir, "mps", name))
+    r = ffi.solve(md, tol=1e-8, precision_bits=64, cpu_threads=1)
+    assert r["status"] == "near_optimal"
+    assert r["objective"] == pytest.approx(optimum, rel=1e-6)

@@ -236,3 +236,13 @@ def test_kernel_event_profile(packing_problem):
     t = dev.kernel_times()
     assert t["row_launches"] == 64 and t["col_launches"] == 64
     assert t["row_ms"] > 0 and t["col_ms"] > 0
+
+
+def test_csr_matvec_counterpart():
+    """python -m taral_pdhg.csr_matvec (gpu/csr_matvec.cu's shapes and fields) on its smallest size: the GPU product
+    matches the CPU loop and the GB/s field follows from the byte count and the median time."""
+    from taral_pdhg import csr_matvec
+    for bits in (32, 64):
+        r = csr_matvec.bench(1024, bits, repeats=3)
+        assert r["max_rel_error"] <= (1e-5 if bits == 32 else 1e-13)
+        assert r["gpu_gbs"] == pytest.approx(r["bytes"] / r["gpu_median_s"] / 1e9)

@@ -84,3 +84,15 @@ def test_iteration_limit_and_json(cfg0):
               "cols", "nnz", "norm_a", "primal_weight", "eval_freq", "graph", "device_mem_bytes"):
         assert k in j, k
     assert j["status"] == "iteration_limit" and j["rows"] == 2000 and j["backend"] == "gpu"
+
+
+@pytest.mark.parametrize("fast", [False, True])
+@pytest.mark.parametrize("name,optimum", [("ref_steel_production.mps", 192000.0), ("ref_transportation.mps", 153.675)])
+def test_literature_lps(golden_dir, name, optimum, fast):
+    """examples/literature_lp/*.mps to the optima of results/literature_lp_20261003/results.json, on the reference
+    scheme (FP64) and on the fast path (FP32 iterates + FP64 refinement)."""
+    md = ffi.Model.from_mps(os.path.join(golden_dir, "mps", name))
+    opts = dict(precision_bits=32, halpern=1, refine=1) if fast else dict(precision_bits=64)
+    r = ffi.solve(md, tol=1e-8, **opts)
+    assert r["status"] == "near_optimal"
+    assert r["objective"] == pytest.approx(optimum, rel=1e-6)
