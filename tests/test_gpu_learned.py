@@ -7,9 +7,12 @@ bf16 MFMA kernels) is trained on the GPU and judged on 128 held-out scenarios ag
 the cold start and the training-mean solution as a fixed warm start. Asserted: the network's predicted points are
 closer to the held-out solutions than the training mean and closer than the untrained network's; their relative KKT
 beats the cold start's for most scenarios; every warm-started solve reaches 1e-6 (checked independently); and the
-fallback sends a useless prediction back to the cold start with the cold start's exact iterations. The iteration
-and solves/sec gains are printed, not asserted: on this family PDHG's iteration count responds weakly to start
-points (even the true solution with 1% noise saves about a third), so they are modest.
+fallback sends a useless prediction back to the cold start with the cold start's exact iterations; and the learned
+start never costs more than 5% mean iterations over the cold start. An iteration gain is printed, not asserted, and
+it is small: on this family PDHG gains from a start only when the start is within a few percent of the solution
+(measured on 64 held-out scenarios of this base: the true solution with 0.01% / 1% / 3% / 10% relative noise gives
+1.98x / 1.51x / 1.28x / 1.07x fewer mean iterations), while the solutions move about 35% between scenarios and the
+network predicts them to about 25% (1.02x; the nearest training scenario's solution, 38% off, 1.04x).
 """
 import numpy as np
 import pytest
@@ -55,6 +58,7 @@ def test_trained_network_beats_the_methods_without_it(family_data):
     print("cold: %.1f solves/s, mean iterations %.0f; learned: %.1f solves/s, mean iterations %.0f, warm used %d"
           % (len(th_te) / s_cold, it_c.mean(), len(th_te) / s_warm, it_w.mean(), sum(r["warm_start_used"] for r in rw)))
     assert np.median([r["warm_kkt"] for r in rw]) < np.median([r["cold_kkt"] for r in rw])
+    assert it_w.mean() <= 1.05 * it_c.mean()
     for t, r in zip(th_te, rw):
         assert r["status"] == "near_optimal"
         assert max(rel_kkt(fam.model(t), r["x"], r["y"])) <= 1.0001e-6
