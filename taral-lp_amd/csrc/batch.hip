@@ -347,8 +347,10 @@ __global__ void k_reset_bar(unsigned* count, int* err) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0u, *err = 0;
 }
 
-// Reduction / elementwise table: (instance, q, nb) entries; entry q of nb covers local indices
-// q * kWG + t, stepping nb * kWG (the single-instance grid-stride partition with nb = red_blocks(size)).
+// Reduction / elementwise table: (instance, q, nb, slot) entries; entry q of nb covers local indices
+// q * kWG + t, stepping nb * kWG (the single-instance grid-stride partition with nb = red_blocks(size)); slot is the
+// entry's index in the whole table, where its partial sums go, so a launch over a compacted list of the live
+// instances' entries fills the same slots as one over the whole table.
 struct Seg {
     const int* tab;
     const int* off;   // first global index of each instance
@@ -356,8 +358,9 @@ struct Seg {
 };
 
 #define SEG_LOOP(S, on, ...)                                                          \
-    const int inst_ = (S).tab[3 * blockIdx.x], q_ = (S).tab[3 * blockIdx.x + 1];      \
-    const int nb_ = (S).tab[3 * blockIdx.x + 2];                                       \
+    const int* e_ = (S).tab + 4 * blockIdx.x;                                          \
+    const int inst_ = e_[0], q_ = e_[1], nb_ = e_[2], slot_ = e_[3];                   \
+    (void)slot_;                                                                       \
     const bool act_ = (on)[inst_] != 0;                                                \
     const int base_ = (S).off[inst_], size_ = (S).size[inst_];                         \
     if (act_)                                                                          \
@@ -415,7 +418,7 @@ __global__ __launch_bounds__(kWG) void k_bsumsq(Seg S, const uint8_t* __restrict
                                                 double* __restrict__ part) {
     double v[1] = {0};
     SEG_LOOP(S, on, { v[0] += double(x[g]) * double(x[g]); })
-    block_store<1>(v, part);
+    block_store<1>(v, part, slot_);
 }
 
 __global__ void k_bset_counters(int B, const uint8_t* __restrict__ on, int* __restrict__ kc, int add, int reset) {
@@ -446,7 +449,7 @@ __global__ __launch_bounds__(kWG) void k_beval_rows(Seg S, const uint8_t* __rest
         const double dz = double(a.z[g]) - double(a.z0[g]);
         v[4] += dz * dz;
     })
-    block_store<5>(v, part);
+    block_store<5>(v, part, slot_);
 }
 
 template <class T>
@@ -468,7 +471,7 @@ __global__ __launch_bounds__(kWG) void k_beval_cols(Seg S, const uint8_t* __rest
         const double ro = res / a.Dc[g], dx = double(a.x[g]) - double(a.x0[g]);
         v[0] += res * res, v[1] += ro * ro, v[4] += dx * dx;
     })
-    block_store<5>(v, part);
+    block_store<5>(v, part, slot_);
 }
 
 // Per instance: out[inst * stride + ooff + k] = sum of its K partials (k_finish per segment, same order).
@@ -564,9 +567,16 @@ public:
         HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         build_host(probs);
         upload_all(probs);
+        const char* pick = o.cap ? nullptr : std::getenv("TPDHG_TUNED_STAGE");
         if (o.cap) {
             set_cap(o.cap);
             tuned_cap_ = o.cap;
+        } else if (pick) {  // the autotuner's pick given (tests of the stage following the live set)
+            const int c = std::atoi(pick);
+            if (c != 1024 && c != 2048 && c != 4096 && c != 8192)
+                throw std::runtime_error("TPDHG_TUNED_STAGE must be 1024, 2048, 4096 or 8192");
+            set_cap(c);
+            tuned_cap_ = c;
         } else {
             autotune();
         }
@@ -675,8 +685,12 @@ public:
         upload_flags(on);
         spmv_flagged(A64_, xb_, BStore<double>{Axb_, 0}, on, true);
         spmv_flagged(AT64_, zb_, BStore<double>{ATzb_, 0}, on, false);
-        k_bbegin_rows<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, rl64_, ru64_, Axb_, zb_, rl_in_, ru_in_, zh_, z_, z0_);
-        k_bbegin_cols<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, c64_, l64_, u64_, ATzb_, xb_, c_in_, l_in_, u_in_,
+        const auto rs = rsegs(on), cs = csegs(on);
+        if (rs.second)
+            k_bbegin_rows<T><<<rs.second, kWG, 0, st_>>>(rs.first, dflags_, rl64_, ru64_, Axb_, zb_, rl_in_, ru_in_, zh_,
+                                                         z_, z0_);
+        if (cs.second)
+            k_bbegin_cols<T><<<cs.second, kWG, 0, st_>>>(cs.first, dflags_, c64_, l64_, u64_, ATzb_, xb_, c_in_, l_in_, u_in_,
                                                   x_, x0_);
         HIP_OK(hipStreamSynchronize(st_));
         HIP_OK(hipGetLastError());
@@ -690,8 +704,9 @@ public:
 
     void restart(const std::vector<uint8_t>& on) override {
         upload_flags(on);
-        k_bcopy<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, x0_, x_);
-        k_bcopy<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, z0_, z_);
+        const auto rs = rsegs(on), cs = csegs(on);
+        if (cs.second) k_bcopy<T><<<cs.second, kWG, 0, st_>>>(cs.first, dflags_, x0_, x_);
+        if (rs.second) k_bcopy<T><<<rs.second, kWG, 0, st_>>>(rs.first, dflags_, z0_, z_);
         k_bset_counters<<<(B_ + kWG - 1) / kWG, kWG, 0, st_>>>(B_, dflags_, kc_, 0, 1);
         spmv_flagged(AT_, z_, col_epi(0, 1, true), on, false);
         HIP_OK(hipStreamSynchronize(st_));
@@ -700,8 +715,9 @@ public:
 
     void end_round(const std::vector<uint8_t>& on) override {
         upload_flags(on);
-        k_baccumulate<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, xb_, x_);
-        k_baccumulate<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, zb_, z_);
+        const auto rs = rsegs(on), cs = csegs(on);
+        if (cs.second) k_baccumulate<T><<<cs.second, kWG, 0, st_>>>(cs.first, dflags_, xb_, x_);
+        if (rs.second) k_baccumulate<T><<<rs.second, kWG, 0, st_>>>(rs.first, dflags_, zb_, z_);
         HIP_OK(hipStreamSynchronize(st_));
         HIP_OK(hipGetLastError());
     }
@@ -741,8 +757,9 @@ public:
         upload_flags(on);
         spmv_flagged(A_, x_, BStore<T>{Ax_, 0}, on, true);
         BEvalArgs<T> a{Ax_, z_, z0_, x_, ATz_, x0_, Axb_, zb_, xb_, ATzb_, rl64_, ru64_, Dr64_, c64_, l64_, u64_, Dc64_};
-        k_beval_rows<T><<<nrseg_, kWG, 0, st_>>>(rseg(), dflags_, a, part_);
-        k_beval_cols<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, a, part_ + size_t(nrseg_) * 5);
+        const auto rs = rsegs(on), cs = csegs(on);
+        if (rs.second) k_beval_rows<T><<<rs.second, kWG, 0, st_>>>(rs.first, dflags_, a, part_);
+        if (cs.second) k_beval_cols<T><<<cs.second, kWG, 0, st_>>>(cs.first, dflags_, a, part_ + size_t(nrseg_) * 5);
         k_bfinish<<<B_, kWG, 0, st_>>>(d_rfirst_, d_rcount_, dflags_, 5, part_, NKKT, 0, out_);
         k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst5_, d_ccount_, dflags_, 5, part_, NKKT, 5, out_);
         HIP_OK(hipMemcpyAsync(hout_, out_, size_t(B_) * NKKT * sizeof(double), hipMemcpyDeviceToHost, st_));
@@ -880,13 +897,16 @@ private:
                        std::vector<int32_t>& count) {
             for (int i = 0; i < B_; ++i) {
                 const int nb = red_blocks(size[size_t(i)]);
-                first.push_back(int32_t(tab.size() / 3)), count.push_back(nb);
-                for (int q = 0; q < nb; ++q) tab.push_back(i), tab.push_back(q), tab.push_back(nb);
+                first.push_back(int32_t(tab.size() / 4)), count.push_back(nb);
+                for (int q = 0; q < nb; ++q) {
+                    const int32_t slot = int32_t(tab.size() / 4);
+                    tab.push_back(i), tab.push_back(q), tab.push_back(nb), tab.push_back(slot);
+                }
             }
         };
         seg(m_, rtab_, rfirst_, rcount_);
         seg(n_, ctab_, cfirst_, ccount_);
-        nrseg_ = int(rfirst_.empty() ? 0 : rtab_.size() / 3), ncseg_ = int(ctab_.size() / 3);
+        nrseg_ = int(rfirst_.empty() ? 0 : rtab_.size() / 4), ncseg_ = int(ctab_.size() / 4);
     }
 
     BMeta* up_meta(const std::vector<BMeta>& v) {
@@ -952,6 +972,7 @@ private:
         d_roff_ = up_int(roff_), d_coff_ = up_int(coff_);
         d_m_ = up_int(m_), d_n_ = up_int(n_);
         d_rtab_ = up_int(rtab_), d_ctab_ = up_int(ctab_);
+        d_rlive_ = mem_.alloc<int>(std::max<size_t>(rtab_.size(), 4)), d_clive_ = mem_.alloc<int>(std::max<size_t>(ctab_.size(), 4));
         d_rfirst_ = up_int(rfirst_), d_rcount_ = up_int(rcount_), d_cfirst_ = up_int(cfirst_), d_ccount_ = up_int(ccount_);
         {
             std::vector<int32_t> c5(cfirst_);
@@ -1231,6 +1252,20 @@ private:
 
     Seg rseg() const { return Seg{d_rtab_, d_roff_, d_m_}; }
     Seg cseg() const { return Seg{d_ctab_, d_coff_, d_n_}; }
+    // Whether the live lists hold every instance flagged in `on`.
+    bool covers(const std::vector<uint8_t>& on) const {
+        bool c = list_inst_.size() == size_t(B_);
+        for (int i = 0; i < B_ && c; ++i) c = !on[size_t(i)] || list_inst_[size_t(i)];
+        return c;
+    }
+    // Row / column entries for a launch over the instances flagged in `on`: the live list when it covers them
+    // (a batch's tail then launches a few workgroups instead of the whole table), else the whole table.
+    std::pair<Seg, int> rsegs(const std::vector<uint8_t>& on) const {
+        return covers(on) ? std::make_pair(Seg{d_rlive_, d_roff_, d_m_}, nrlive_) : std::make_pair(rseg(), nrseg_);
+    }
+    std::pair<Seg, int> csegs(const std::vector<uint8_t>& on) const {
+        return covers(on) ? std::make_pair(Seg{d_clive_, d_coff_, d_n_}, nclive_) : std::make_pair(cseg(), ncseg_);
+    }
 
     void upload_flags(const std::vector<uint8_t>& on) {
         std::copy(on.begin(), on.end(), hflags_);
@@ -1273,9 +1308,7 @@ private:
     // the side's whole table. a_side: the list of A (else of A^T).
     template <class S, class U, class Epi>
     void spmv_flagged(const S& side, const U* v, const Epi& epi, const std::vector<uint8_t>& on, bool a_side) {
-        bool cover = list_inst_.size() == size_t(B_);
-        for (int i = 0; i < B_ && cover; ++i) cover = !on[size_t(i)] || list_inst_[size_t(i)];
-        if (cover) spmv_rows(side, v, epi, dflags_, a_side ? alist_ : tlist_, a_side ? nalist_ : ntlist_);
+        if (covers(on)) spmv_rows(side, v, epi, dflags_, a_side ? alist_ : tlist_, a_side ? nalist_ : ntlist_);
         else spmv_rows(side, v, epi, dflags_);
     }
 
@@ -1317,6 +1350,18 @@ private:
             HIP_OK(hipMemcpyAsync(tlist_, hlist_ + la.size(), lt.size() * sizeof(BMeta), hipMemcpyHostToDevice, st_));
         HIP_OK(hipStreamSynchronize(st_));
         nalist_ = int(la.size()), ntlist_ = int(lt.size());
+        {  // the live instances' reduction / elementwise entries (their slots unchanged)
+            std::vector<int32_t> rl, cl;
+            for (int i = 0; i < B_; ++i) {
+                if (!list_inst_[size_t(i)]) continue;
+                rl.insert(rl.end(), rtab_.begin() + 4 * rfirst_[size_t(i)], rtab_.begin() + 4 * (rfirst_[size_t(i)] + rcount_[size_t(i)]));
+                cl.insert(cl.end(), ctab_.begin() + 4 * cfirst_[size_t(i)], ctab_.begin() + 4 * (cfirst_[size_t(i)] + ccount_[size_t(i)]));
+            }
+            if (!rl.empty()) HIP_OK(hipMemcpyAsync(d_rlive_, rl.data(), rl.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+            if (!cl.empty()) HIP_OK(hipMemcpyAsync(d_clive_, cl.data(), cl.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+            nrlive_ = int(rl.size() / 4), nclive_ = int(cl.size() / 4);
+        }
         ++list_builds_;
         if (graph_) {
             (void)hipGraphExecDestroy(graph_);
@@ -1418,6 +1463,8 @@ private:
     Side64 A64_, AT64_;
     int *d_roff_ = nullptr, *d_coff_ = nullptr, *d_m_ = nullptr, *d_n_ = nullptr;
     int *d_rtab_ = nullptr, *d_ctab_ = nullptr, *d_rfirst_ = nullptr, *d_rcount_ = nullptr;
+    int *d_rlive_ = nullptr, *d_clive_ = nullptr;
+    int nrlive_ = 0, nclive_ = 0;
     int *d_cfirst_ = nullptr, *d_cfirst5_ = nullptr, *d_ccount_ = nullptr;
     double *c64_ = nullptr, *l64_ = nullptr, *u64_ = nullptr, *Dc64_ = nullptr, *rl64_ = nullptr, *ru64_ = nullptr,
            *Dr64_ = nullptr;

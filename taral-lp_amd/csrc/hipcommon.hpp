@@ -326,9 +326,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Workgroup sum of K doubles per thread into part[blockIdx * K + k] (lane order fixed: deterministic).
+// Workgroup sum of K doubles per thread into part[slot * K + k], slot = blockIdx unless given (lane order fixed:
+// deterministic).
 template <int K>
-__device__ __forceinline__ void block_store(double (&v)[K], double* __restrict__ part) {
+__device__ __forceinline__ void block_store(double (&v)[K], double* __restrict__ part, int slot = -1) {
     __shared__ double sh[K][kWG / 64];
     const int lane = int(threadIdx.x) & 63, w = int(threadIdx.x) >> 6;
     for (int k = 0; k < K; ++k) {
@@ -340,7 +341,7 @@ __device__ __forceinline__ void block_store(double (&v)[K], double* __restrict__
         const int k = int(threadIdx.x);
         double s = 0;
         for (int q = 0; q < kWG / 64; ++q) s += sh[k][q];
-        part[size_t(blockIdx.x) * K + k] = s;
+        part[size_t(slot < 0 ? int(blockIdx.x) : slot) * K + k] = s;
     }
 }
 
