@@ -1,7 +1,7 @@
 // Warm-start network of the learned path: a 3-layer MLP (n_in -> h1 -> h2 -> n_out, ReLU) trained and run on the
-// GPU with bf16 MFMA matrix products (v_mfma_f32_16x16x16_bf16: bf16 operands, FP32 accumulation), FP32 master
-// weights and Adam. Every matrix product of the forward and backward passes is one launch of k_gemm_bf16, one
-// wave per 16 x 16 output tile, four waves per workgroup. Shapes are padded to multiples of 16 (batch, widths).
+// GPU with bf16 MFMA matrix products (v_mfma_f32_16x16x32_bf16: bf16 operands, FP32 accumulation), FP32 master
+// weights and Adam. Every matrix product of the forward and backward passes is one launch of k_gemm_bf16: 64 x 64
+// output tiles staged through LDS, four waves per workgroup. Shapes are padded to multiples of 16 (batch, widths).
 //
 // Where bf16 rounding happens (the CPU oracle rounds at the same places): inputs X, weight copies, hidden
 // activations H1 and H2, and the back-propagated dY, dZ2, dZ1 before each product; biases, pre-activations,
@@ -19,7 +19,6 @@
 namespace tpdhg {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {  // round to nearest even (finite inputs)
@@ -28,37 +27,102 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {  // round to nearest even
     return uint16_t(u >> 16);
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kTM = 64, kTN = 64, kTK = 32;  // workgroup tile (M x N) and K step
+constexpr int kLdsK = kTK + 8;                 // LDS row length in bf16: 80-byte rows spread a wave's 16-byte reads
+
+// The 8 bf16 of operand X at rows/cols [p0, p0 + 8) of one K step, as one thread's share of an LDS tile load.
+// X(p, k) = X[p * s_p + k * s_k]; with s_k == 1 the thread takes 8 consecutive k of one row p (contiguous in
+// memory), else 8 consecutive p at one k (contiguous when s_p == 1), so a wave's loads are coalesced either way.
+struct TileLoad {
+    uint16_t v[8];
+    int p, k;       // first element within the tile
+    bool along_k;   // the 8 elements run along k (else along p)
+};
+__device__ __forceinline__ void tile_fetch(TileLoad& t, const uint16_t* __restrict__ X, int s_p, int s_k, int p0,
+                                           int k0, int P, int K) {
+    const int tid = int(threadIdx.x);
+    t.along_k = s_k == 1;
+    if (t.along_k) t.p = tid >> 2, t.k = (tid & 3) * 8;  // 64 rows x 4 chunks of 8 k
+    else t.k = tid >> 3, t.p = (tid & 7) * 8;            // 32 k x 8 chunks of 8 rows
+    const int gp = p0 + t.p, gk = k0 + t.k;
+    // P and K are multiples of 16 and chunks start at multiples of 8: a chunk is wholly inside or outside
+    const bool in = gp < P && gk < K;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int dp = t.along_k ? 0 : j, dk = t.along_k ? j : 0;
+        t.v[j] = in ? X[size_t(gp + dp) * size_t(s_p) + size_t(gk + dk) * size_t(s_k)] : uint16_t(0);
+    }
+}
+// Stores the fetched chunk into the LDS tile sh[p][k] (k-contiguous rows of kLdsK).
+__device__ __forceinline__ void tile_store(const TileLoad& t, uint16_t (*sh)[kLdsK]) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (t.along_k) sh[t.p][t.k + j] = t.v[j];
+        else sh[t.p + j][t.k] = t.v[j];
+    }
+}
+
 // C[M x N] = alpha * op(A)[M x K] op(B)[K x N] (+ bias[n]) (ReLU), A(i, k) = A[i * sa_m + k * sa_k],
 // B(k, j) = B[k * sb_k + j * sb_n]; FP32 result to C (row-major, ldc) and, when Cb is given, a bf16 copy.
-// M, N, K multiples of 16; grid (N/32, M/32) rounded up, waves outside the matrix exit.
-// MFMA 16x16x16 bf16 lane layout: lane l holds A(row l%16, k 4(l/16)..+3), B(k 4(l/16)..+3, col l%16) and
+// M, N, K multiples of 16; grid (ceil(N/64), ceil(M/64)), 256 threads. Each workgroup stages 64 x 32 tiles of A
+// and 32 x 64 tiles of B (both stored k-contiguous) through LDS per K step, the next step's global loads issued
+// before the current step's products; each of its four waves owns a 32 x 32 quarter of the 64 x 64 output tile as
+// 2 x 2 v_mfma_f32_16x16x32_bf16 tiles (bf16 operands, FP32 accumulation, K = 32 per instruction).
+// MFMA 16x16x32 bf16 lane layout: lane l holds A(row l%16, k 8(l/16)..+7), B(k 8(l/16)..+7, col l%16) and
 // D(rows 4(l/16)..+3, col l%16).
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, int sa_m,
                                                    int sa_k, const uint16_t* __restrict__ B, int sb_k, int sb_n,
                                                    float alpha, const float* __restrict__ bias, int relu,
                                                    float* __restrict__ C, int ldc, uint16_t* __restrict__ Cb) {
+    __shared__ __align__(16) uint16_t As[kTM][kLdsK];  // As[i][k]
+    __shared__ __align__(16) uint16_t Bs[kTN][kLdsK];  // Bs[j][k]
     const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
-    const int tm = int(blockIdx.y) * 2 + (wave >> 1), tn = int(blockIdx.x) * 2 + (wave & 1);
-    if (tm * 16 >= M || tn * 16 >= N) return;
-    const int r = tm * 16 + (lane & 15), c = tn * 16 + (lane & 15), kq = 4 * (lane >> 4);
-    floatx4 acc = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 a, b;
-        for (int j = 0; j < 4; ++j) {
-            const int k = k0 + kq + j;
-            a[j] = short(A[size_t(r) * size_t(sa_m) + size_t(k) * size_t(sa_k)]);
-            b[j] = short(B[size_t(k) * size_t(sb_k) + size_t(c) * size_t(sb_n)]);
+    const int m0 = int(blockIdx.y) * kTM, n0 = int(blockIdx.x) * kTN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // the wave's quarter of the tile
+    const int li = lane & 15, lk = 8 * (lane >> 4);
+    floatx4 acc[2][2] = {};
+    TileLoad ta, tb;
+    tile_fetch(ta, A, sa_m, sa_k, m0, 0, M, K);
+    tile_fetch(tb, B, sb_n, sb_k, n0, 0, N, K);  // B^T(j, k) = B[j * sb_n + k * sb_k]
+    for (int k0 = 0; k0 < K; k0 += kTK) {
+        __syncthreads();  // the previous step's LDS reads are done
+        tile_store(ta, As), tile_store(tb, Bs);
+        __syncthreads();
+        if (k0 + kTK < K) {
+            tile_fetch(ta, A, sa_m, sa_k, m0, k0 + kTK, M, K);
+            tile_fetch(tb, B, sb_n, sb_k, n0, k0 + kTK, N, K);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            a[t] = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * t + li][lk]);
+            b[t] = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * t + li][lk]);
+        }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
     }
-    for (int j = 0; j < 4; ++j) {
-        const int row = tm * 16 + kq + j;
-        float v = alpha * acc[j];
-        if (bias) v += bias[c];
-        if (relu) v = v > 0.f ? v : 0.f;
-        C[size_t(row) * size_t(ldc) + size_t(c)] = v;
-        if (Cb) Cb[size_t(row) * size_t(ldc) + size_t(c)] = to_bf16(v);
-    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+            const int c = n0 + wn + 16 * tj + li;
+            if (c >= N) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = m0 + wm + 16 * ti + 4 * (lane >> 4) + j;
+                if (row >= M) continue;
+                float v = alpha * acc[ti][tj][j];
+                if (bias) v += bias[c];
+                if (relu) v = v > 0.f ? v : 0.f;
+                C[size_t(row) * size_t(ldc) + size_t(c)] = v;
+                if (Cb) Cb[size_t(row) * size_t(ldc) + size_t(c)] = to_bf16(v);
+            }
+        }
 }
 
 __global__ void k_to_bf16(int64_t k, const float* __restrict__ src, uint16_t* __restrict__ dst) {
@@ -275,7 +339,7 @@ private:
     }
     void gemm(int M, int N, int K, const uint16_t* A, int sa_m, int sa_k, const uint16_t* B, int sb_k, int sb_n,
               const float* bias, int relu, float* C, int ldc, uint16_t* Cb) {
-        dim3 grid((N + 31) / 32, (M + 31) / 32);
+        dim3 grid((N + kTN - 1) / kTN, (M + kTM - 1) / kTM);
         k_gemm_bf16<<<grid, 256, 0, st_>>>(M, N, K, A, sa_m, sa_k, B, sb_k, sb_n, 1.f, bias, relu, C, ldc, Cb);
     }
     void run_forward(int bp) {
