@@ -166,8 +166,11 @@ __device__ __forceinline__ void bblock(const BMeta& m, const int* __restrict__ p
                 if (r < r1) pa[q] = P[r] - k0, pe[q] = P[r + 1] - k0, pp[q] = epi.pre(r);
             }
         }
-        gather_rounds<WG, 8>(k0, k1, idx, val, vi, [](int c) { return uint32_t(c) & kMask; },
-                             [&](int, int c, T x) { sh[uint32_t(c) >> kBatchColBits] = x; });
+        // slots per thread per round: CAP / WG when that is under 8 (the 1,024-nonzero stage: 4, not 8 of which
+        // half would load a clamped duplicate), else 8
+        gather_rounds<WG, (CAP / WG < 8 ? CAP / WG : 8)>(k0, k1, idx, val, vi,
+                                                         [](int c) { return uint32_t(c) & kMask; },
+                                                         [&](int, int c, T x) { sh[uint32_t(c) >> kBatchColBits] = x; });
         __syncthreads();
         if (grouped) {
             T s = 0;
