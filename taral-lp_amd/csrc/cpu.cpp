@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cmath>
 #include <condition_variable>
+#include <exception>
 #include <cstdio>
 #include <functional>
 #include <mutex>
@@ -51,8 +52,19 @@ public:
             gen_.fetch_add(1, std::memory_order_release);
         }
         cv_.notify_all();
-        f(0);
-        while (left_.load(std::memory_order_acquire) != 0) std::this_thread::yield();
+        std::exception_ptr mine;
+        try {
+            f(0);
+        } catch (...) {
+            mine = std::current_exception();
+        }
+        while (left_.load(std::memory_order_acquire) != 0) std::this_thread::yield();  // f outlives every use
+        if (mine) std::rethrow_exception(mine);
+        if (err_) {  // a worker's exception, rethrown on the calling thread
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
     }
 
 private:
@@ -68,7 +80,12 @@ private:
             }
             seen = g;
             if (stop_) return;
-            (*job_)(t);
+            try {
+                (*job_)(t);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!err_) err_ = std::current_exception();
+            }
             left_.fetch_sub(1, std::memory_order_acq_rel);
         }
     }
@@ -79,6 +96,7 @@ private:
     std::atomic<unsigned> gen_{0};
     std::atomic<int> left_{0};
     const std::function<void(int)>* job_ = nullptr;
+    std::exception_ptr err_;  // first exception of a worker in the current run (written under mu_)
     bool stop_ = false;
 };
 

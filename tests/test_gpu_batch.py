@@ -198,3 +198,16 @@ def test_stage_follows_the_live_set(monkeypatch):
     _same(r1, r2)
     for md, r in zip(lps, r1):
         assert r["status"] == "near_optimal" and max(rel_kkt(md, r["x"], r["y"])) <= 1.0001e-6
+
+
+def test_probe_leaves_later_solves_unchanged():
+    """tpdhg_batch_probe (the benchmark's roofline timing) iterates all instances on the batch's own device vectors;
+    every solve re-initialises them, so a solve after a probe repeats the solve before it bit for bit. The probe
+    reports one row and one column launch per iteration, each timed."""
+    b = ffi.Batch([ffi.Model.from_lp(md) for md in _mixed()], **FAST)
+    r1 = b.solve()
+    p = b.probe(8)
+    r2 = b.solve()
+    b.close()
+    assert p["row_launches"] == 8 and p["col_launches"] == 8 and p["row_ms"] > 0 and p["col_ms"] > 0
+    _same(r1, r2)
