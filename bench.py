@@ -376,8 +376,10 @@ def run_batch(a, ffi, gen, desc, lps, budget):
         conv = sum(st == "near_optimal" for st, _, _ in fp64)
         rate = len(fp64) / tot
         out["fp64_reference"] = dict(
-            sample=f"{len(fp64)} of {len(lps)} LPs, one from each nonzero-count slice (stratified, seed 7), solved "
-                   f"one at a time (each on its own resident session; power iteration included)",
+            sample=(f"all {len(lps)} LPs" if len(sample) == len(lps) else
+                    f"{len(sample)} of {len(lps)} LPs, one from each nonzero-count slice (stratified, seed 7)") +
+                   (f" ({len(fp64)} solved before the leg budget ran out)" if len(fp64) < len(sample) else "") +
+                   ", solved one at a time (each on its own resident session; power iteration included)",
             lps=[lps[i].name for i in done], converged=conv, wall_s=round(tot, 3), solves_per_s=round(rate, 4),
             time_cap_s=a.fp64_time_limit,
             note="solves that hit the cap count at the cap, so the FP64 rate is an upper bound when converged < sample")
