@@ -141,6 +141,11 @@ int tpdhg_fast_iterate(tpdhg_fast* f, int32_t steps);
 /* KKT scalars (slots as tpdhg_dev_eval) of base + inner on the original scaled problem, FP64; DX2/DZ2 measure
  * the distance from the anchors. */
 int tpdhg_fast_eval(tpdhg_fast* f, double out[10]);
+/* Fixed-point residual terms of the inner point u = (x, z) at the last tpdhg_fast_eval, FP64 sums: out[0] =
+ * ||x - x+||^2, out[1] = ||z - z+||^2, out[2] = (z - z+)' A (x - x+), where (x+, z+) is one PDHG step from u:
+ * x+ = proj(x - tau (c_in + A^T z)), z+ = prox(z + sigma A (2 x+ - x)). ||u - T u||_P^2 = out[0] / tau +
+ * out[1] / sigma + 2 out[2] is what tpdhg_options.restart_metric = 1 measures. */
+int tpdhg_fast_fixed_point(tpdhg_fast* f, double out[3]);
 /* base += inner point. */
 int tpdhg_fast_end_round(tpdhg_fast* f);
 /* base + inner point (FP64, scaled space); either may be NULL. */
@@ -180,6 +185,12 @@ typedef struct {
                             --device cpu --threads N, gpu/pdhg.cu:517-741, 913-934) with this many host threads,
                             0 = the reference's choice (1 below 200,000 nonzeros, else every hardware thread);
                             results do not depend on the thread count. halpern and refine must be 0 */
+    int32_t restart_metric; /* fast path and batches: what the restart tests (sufficient decay 0.2, necessary decay
+                               0.8 without progress, artificial at 36% of the iterations) measure. 1 (default): the
+                               fixed-point residual ||u - T u||_P of the current point u = (x, z), T one PDHG step,
+                               in the PDHG norm ||dx||^2 / tau + ||dz||^2 / sigma + 2 dz' A dx at the current weight;
+                               0: the reference's weighted KKT error (gpu/pdhg.cu:783-786). The reference scheme
+                               (halpern = 0) always uses the latter. No reference counterpart */
 } tpdhg_options;
 void tpdhg_options_default(tpdhg_options* o);
 

@@ -28,6 +28,7 @@ namespace {
 // Sorted pools pack (row-order position << kBatchColBits) | local column: instances need fewer than 2^19 columns
 // (and rows), and a block holds at most 2^13 nonzeros.
 constexpr int kBatchColBits = 19;
+constexpr int kEvalK = 7;  // partial sums per evaluation block: 5 KKT slots and 2 fixed-point residual terms
 
 template <class T>
 __device__ __forceinline__ T batch_lambda(int k) {
@@ -434,14 +435,19 @@ struct BEvalArgs {
     const T *Ax, *z, *z0, *x, *ATz, *x0;
     const double *Axb, *zb, *xb, *ATzb;
     const double *rl, *ru, *Dr, *c, *l, *u, *Dc;
+    const T *Aw, *xp, *zh, *rl_in, *ru_in, *par;  // fixed-point residual: A w, x+, the inner row prox data
 };
 
-// KKT slots of base + inner per instance (fast.hip k_eval_fast, per segment).
+// KKT slots of base + inner per instance and the fixed-point residual terms of the inner point (fast.hip
+// k_eval_fast, per segment, the same seven partials in the same order).
 template <class T>
 __global__ __launch_bounds__(kWG) void k_beval_rows(Seg S, const uint8_t* __restrict__ on, BEvalArgs<T> a,
                                                     double* __restrict__ part) {
-    double v[5] = {0, 0, 0, 0, 0};
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
     SEG_LOOP(S, on, {
+        const T zp = row_prox(a.z[g], a.Aw[g], a.zh[g], a.rl_in[g], a.ru_in[g], a.par[2 * inst_ + 1]);
+        const double dzt = double(a.z[g]) - double(zp), adx = 0.5 * (double(a.Ax[g]) - double(a.Aw[g]));
+        v[5] += dzt * dzt, v[6] += dzt * adx;
         const double ax = a.Axb[g] + double(a.Ax[g]), zz = a.zb[g] + double(a.z[g]);
         const double lo = a.rl[g], up = a.ru[g], dr = a.Dr[g];
         const double r = ax < lo ? lo - ax : (ax > up ? ax - up : 0), ro = r / dr;
@@ -452,13 +458,13 @@ __global__ __launch_bounds__(kWG) void k_beval_rows(Seg S, const uint8_t* __rest
         const double dz = double(a.z[g]) - double(a.z0[g]);
         v[4] += dz * dz;
     })
-    block_store<5>(v, part, slot_);
+    block_store<7>(v, part, slot_);
 }
 
 template <class T>
 __global__ __launch_bounds__(kWG) void k_beval_cols(Seg S, const uint8_t* __restrict__ on, BEvalArgs<T> a,
                                                     double* __restrict__ part) {
-    double v[5] = {0, 0, 0, 0, 0};
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
     SEG_LOOP(S, on, {
         const double xx = a.xb[g] + double(a.x[g]), cj = a.c[g];
         const double d = cj + a.ATzb[g] + double(a.ATz[g]), lo = a.l[g], up = a.u[g];
@@ -473,8 +479,10 @@ __global__ __launch_bounds__(kWG) void k_beval_cols(Seg S, const uint8_t* __rest
         }
         const double ro = res / a.Dc[g], dx = double(a.x[g]) - double(a.x0[g]);
         v[0] += res * res, v[1] += ro * ro, v[4] += dx * dx;
+        const double dxt = double(a.x[g]) - double(a.xp[g]);
+        v[5] += dxt * dxt;
     })
-    block_store<5>(v, part, slot_);
+    block_store<7>(v, part, slot_);
 }
 
 // Per instance: out[inst * stride + ooff + k] = sum of its K partials (k_finish per segment, same order).
@@ -759,20 +767,31 @@ public:
     void eval(const std::vector<uint8_t>& on, std::vector<double>& out) override {
         upload_flags(on);
         spmv_flagged(A_, x_, BStore<T>{Ax_, 0}, on, true);
-        BEvalArgs<T> a{Ax_, z_, z0_, x_, ATz_, x0_, Axb_, zb_, xb_, ATzb_, rl64_, ru64_, Dr64_, c64_, l64_, u64_, Dc64_};
+        spmv_flagged(A_, w_, BStore<T>{pw_, 0}, on, true);  // A w, w = 2 x+ - x: the dual half of T(u)
+        BEvalArgs<T> a{Ax_, z_, z0_, x_, ATz_, x0_, Axb_, zb_, xb_, ATzb_, rl64_, ru64_, Dr64_, c64_, l64_, u64_, Dc64_,
+                       pw_, xp_, zh_, rl_in_, ru_in_, par_};
         const auto rs = rsegs(on), cs = csegs(on);
         if (rs.second) k_beval_rows<T><<<rs.second, kWG, 0, st_>>>(rs.first, dflags_, a, part_);
-        if (cs.second) k_beval_cols<T><<<cs.second, kWG, 0, st_>>>(cs.first, dflags_, a, part_ + size_t(nrseg_) * 5);
-        k_bfinish<<<B_, kWG, 0, st_>>>(d_rfirst_, d_rcount_, dflags_, 5, part_, NKKT, 0, out_);
-        k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst5_, d_ccount_, dflags_, 5, part_, NKKT, 5, out_);
-        HIP_OK(hipMemcpyAsync(hout_, out_, size_t(B_) * NKKT * sizeof(double), hipMemcpyDeviceToHost, st_));
+        if (cs.second)
+            k_beval_cols<T><<<cs.second, kWG, 0, st_>>>(cs.first, dflags_, a, part_ + size_t(nrseg_) * kEvalK);
+        k_bfinish<<<B_, kWG, 0, st_>>>(d_rfirst_, d_rcount_, dflags_, kEvalK, part_, 2 * kEvalK, 0, out_);
+        k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst5_, d_ccount_, dflags_, kEvalK, part_, 2 * kEvalK, kEvalK, out_);
+        HIP_OK(hipMemcpyAsync(hout_, out_, size_t(B_) * 2 * kEvalK * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIP_OK(hipStreamSynchronize(st_));
         HIP_OK(hipGetLastError());
         out.resize(size_t(B_) * NKKT);
-        for (int i = 0; i < B_; ++i)
-            if (on[size_t(i)]) std::copy(hout_ + size_t(i) * NKKT, hout_ + size_t(i + 1) * NKKT, out.begin() + size_t(i) * NKKT);
+        fp_.resize(size_t(B_) * 3);
+        for (int i = 0; i < B_; ++i) {
+            if (!on[size_t(i)]) continue;
+            const double* h = hout_ + size_t(i) * 2 * kEvalK;  // rows: 5 KKT slots + 2, columns: 5 + 2
+            std::copy(h, h + 5, out.begin() + size_t(i) * NKKT);
+            std::copy(h + kEvalK, h + kEvalK + 5, out.begin() + size_t(i) * NKKT + 5);
+            fp_[size_t(i) * 3] = h[kEvalK + 5], fp_[size_t(i) * 3 + 1] = h[5], fp_[size_t(i) * 3 + 2] = h[6];
+        }
         collect_events();
     }
+
+    void fixed_points(std::vector<double>& out) const override { out = fp_; }
 
     void download(int i, double* hx, double* hz) override {
         const int n = n_[size_t(i)], m = m_[size_t(i)];
@@ -998,14 +1017,15 @@ private:
         kc_ = mem_.alloc<int>(size_t(B_));
         HIP_OK(hipMemsetAsync(kc_, 0, size_t(B_) * sizeof(int), st_));
         fac_ = zeros64(B_);
-        part_ = mem_.alloc<double>(size_t(std::max(1, nrseg_ + ncseg_)) * 5);
-        out_ = zeros64(B_ * NKKT);
+        part_ = mem_.alloc<double>(size_t(std::max(1, nrseg_ + ncseg_)) * kEvalK);
+        out_ = zeros64(B_ * 2 * kEvalK);
         dflags_ = mem_.alloc<uint8_t>(size_t(B_)), dact_ = mem_.alloc<uint8_t>(size_t(B_));
         HIP_OK(hipMemsetAsync(dflags_, 0, size_t(B_), st_));
         HIP_OK(hipMemsetAsync(dact_, 0, size_t(B_), st_));
         HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hflags_), size_t(B_), hipHostMallocDefault));
         HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hpar_), size_t(2 * B_) * sizeof(T), hipHostMallocDefault));
-        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hout_), size_t(B_) * NKKT * sizeof(double), hipHostMallocDefault));
+        HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hout_), size_t(B_) * 2 * kEvalK * sizeof(double),
+                             hipHostMallocDefault));
         HIP_OK(hipStreamSynchronize(st_));
     }
 
@@ -1478,6 +1498,7 @@ private:
     T *pv_ = nullptr, *pw_ = nullptr, *pu_ = nullptr, *par_ = nullptr, *hpar_ = nullptr;
     int* kc_ = nullptr;
     double *part_ = nullptr, *out_ = nullptr, *hout_ = nullptr;
+    std::vector<double> fp_;  // fixed-point residual terms of the last eval, 3 per instance
     uint8_t *dflags_ = nullptr, *dact_ = nullptr, *hflags_ = nullptr;
     size_t mem_used_ = 0;
     std::string name_;

@@ -37,6 +37,9 @@ public:
     virtual void iterate(int steps, const std::vector<uint8_t>& on) = 0;
     // KKT slots (NKKT per instance, instance-major) of the flagged instances; other rows are left as they were.
     virtual void eval(const std::vector<uint8_t>& on, std::vector<double>& out) = 0;
+    // Per instance (3 entries each, valid for the instances of the last eval): the fixed-point residual terms of
+    // FastDevice::fixed_point.
+    virtual void fixed_points(std::vector<double>& out) const = 0;
     // base + inner point of instance i in FP64, scaled space.
     virtual void download(int i, double* x, double* z) = 0;
     // New scaled c, l, u, rl, ru of instance i (its matrix and scaling stay).

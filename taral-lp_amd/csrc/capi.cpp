@@ -84,6 +84,7 @@ tpdhg::Options to_options(const tpdhg_options* o) {
     r.omega_rule = o->omega_rule;
     r.reuse_norm = o->reuse_norm != 0;
     r.cpu_threads = o->cpu_threads;
+    r.restart_metric = o->restart_metric;
     return r;
 }
 
@@ -353,6 +354,10 @@ int tpdhg_fast_eval(tpdhg_fast* f, double out[10]) {
     if (!out) return fail(TPDHG_ERR_ARG, "null output");
     FAST_CALL(f, f->d->eval(out));
 }
+int tpdhg_fast_fixed_point(tpdhg_fast* f, double out[3]) {
+    if (!out) return fail(TPDHG_ERR_ARG, "null output");
+    FAST_CALL(f, f->d->fixed_point(out));
+}
 int tpdhg_fast_end_round(tpdhg_fast* f) { FAST_CALL(f, f->d->end_round()); }
 int tpdhg_fast_download(tpdhg_fast* f, double* x, double* z) { FAST_CALL(f, f->d->download(x, z)); }
 void tpdhg_fast_destroy(tpdhg_fast* f) { delete f; }
@@ -369,6 +374,7 @@ void tpdhg_options_default(tpdhg_options* o) {
     o->omega_rule = d.omega_rule;
     o->reuse_norm = d.reuse_norm ? 1 : 0;
     o->cpu_threads = d.cpu_threads;
+    o->restart_metric = d.restart_metric;
 }
 
 namespace {
@@ -402,6 +408,7 @@ int check_options(const tpdhg::Options& o) {
     if (o.halpern && !(o.rho >= 1 && o.rho < 2)) return fail(TPDHG_ERR_ARG, "reflection must be in [1, 2)");
     if (!(o.omega0 >= 0) || !std::isfinite(o.omega0)) return fail(TPDHG_ERR_ARG, "primal_weight must be finite and >= 0");
     if (o.omega_rule < 0 || o.omega_rule > 3) return fail(TPDHG_ERR_ARG, "omega_rule must be 0, 1, 2 or 3");
+    if (o.restart_metric < 0 || o.restart_metric > 1) return fail(TPDHG_ERR_ARG, "restart_metric must be 0 or 1");
     if (o.cpu_threads < -1 || o.cpu_threads > 4096) return fail(TPDHG_ERR_ARG, "cpu_threads must be -1 (GPU) or 0..4096");
     if (o.cpu_threads >= 0 && o.halpern)
         return fail(TPDHG_ERR_ARG, "the CPU backend runs the reference scheme: set halpern = 0 and refine = 0");

@@ -97,6 +97,10 @@ public:
     virtual void restart() = 0;
     virtual void iterate(int steps) = 0;
     virtual void eval(double out[NKKT]) = 0;
+    // Fixed-point residual of the last eval's inner point u = (x, z) under one iteration T of the (unreflected,
+    // unanchored) PDHG step: out[0] = ||x - x+||^2, out[1] = ||z - z+||^2, out[2] = (z - z+)' A (x - x+), FP64 sums;
+    // ||u - T u||_P^2 = out[0] / tau + out[1] / sigma + 2 out[2] in the norm of the PDHG metric.
+    virtual void fixed_point(double out[3]) const = 0;
     // base += inner point (FP64).
     virtual void end_round() = 0;
     // base + inner point in FP64, scaled space; either may be null.

@@ -182,7 +182,16 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
     if (omega0 > 0) w = omega0;
     const double w_start = w;
     b.set_base(x0, z0);
+    // The restart tests' measure of the current point (restart_metric): the fixed-point residual of the last
+    // evaluation, ||u - T u||_P = sqrt(||dx||^2 / tau + ||dz||^2 / sigma + 2 dz' A dx) with tau = eta / w and
+    // sigma = eta w (the norm in which a PDHG step is nonexpansive; Halpern iterations drive it down at a known
+    // rate between restarts), or the reference's weighted KKT error.
     auto wkkt = [&](const double* k) {
+        if (o.restart_metric == 1) {
+            double f[3];
+            b.fixed_point(f);
+            return std::sqrt(std::max(0.0, f[0] * w / eta + f[1] / (eta * w) + 2 * f[2]));
+        }
         const double g = k[POBJ] - (k[DROW] + k[DCOL]);
         return std::sqrt(w * k[PR2S] + k[DR2S] / w + g * g);
     };
@@ -424,23 +433,31 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
         b.set_steps(tau, sig);
     };
     auto kslot = [](std::vector<double>& kk, int i) { return kk.data() + size_t(i) * NKKT; };
-    auto wkkt = [&](const double* k, double wi) {
-        const double g = k[POBJ] - (k[DROW] + k[DCOL]);
-        return std::sqrt(wi * k[PR2S] + k[DR2S] / wi + g * g);
-    };
     b.set_base(x0, z0);
     std::vector<uint8_t> all(static_cast<size_t>(B), 1), active(static_cast<size_t>(B), 0);
     std::vector<double> kk(size_t(B) * NKKT, 0.0), k_last(static_cast<size_t>(B)), k_prev(static_cast<size_t>(B), kInf);
+    std::vector<double> fps;  // fixed-point residual terms of the last evaluation, 3 per instance
+    auto wkkt = [&](int i, double wi) {  // run_fast's restart measure for instance i at weight wi
+        if (o.restart_metric == 1) {
+            const double ei = eta[size_t(i)];
+            const double* f = fps.data() + 3 * size_t(i);
+            return std::sqrt(std::max(0.0, f[0] * wi / ei + f[1] / (ei * wi) + 2 * f[2]));
+        }
+        const double* k = kslot(kk, i);
+        const double g = k[POBJ] - (k[DROW] + k[DCOL]);
+        return std::sqrt(wi * k[PR2S] + k[DR2S] / wi + g * g);
+    };
     std::vector<int64_t> n_since(static_cast<size_t>(B), 0), it_of(static_cast<size_t>(B), 0);
     std::vector<Rel> best(static_cast<size_t>(B), Rel{kInf, kInf, kInf});
     b.begin_round(all);
     push_steps();
     b.restart(all);
     b.eval(all, kk);
+    b.fixed_points(fps);
     int live = 0;
     for (int i = 0; i < B; ++i) {
         best[size_t(i)] = rel_of(kslot(kk, i), *P[size_t(i)], o.row_rel);
-        k_last[size_t(i)] = wkkt(kslot(kk, i), w[size_t(i)]);
+        k_last[size_t(i)] = wkkt(i, w[size_t(i)]);
         if (best[size_t(i)].worst() <= o.tol) r[size_t(i)].status = "near_optimal";
         else active[size_t(i)] = 1, ++live;
     }
@@ -452,6 +469,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
         b.iterate(steps, active);
         it += steps;
         b.eval(active, kk);
+        b.fixed_points(fps);
         std::fill(confirm.begin(), confirm.end(), 0), std::fill(rs_move.begin(), rs_move.end(), 0);
         std::fill(rs_any.begin(), rs_any.end(), 0);
         bool any_confirm = false, any_move = false, any_rs = false;
@@ -479,7 +497,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
                 active[size_t(i)] = 0, --live;
                 continue;
             }
-            const double kc = wkkt(k, w[size_t(i)]);
+            const double kc = wkkt(i, w[size_t(i)]);
             const bool restart = double(n_since[size_t(i)]) >= 0.36 * double(it) || kc <= 0.2 * k_last[size_t(i)] ||
                                  (kc <= 0.8 * k_last[size_t(i)] && kc > k_prev[size_t(i)]);
             k_prev[size_t(i)] = kc;
@@ -497,6 +515,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
             push_steps();
             b.restart(confirm);
             b.eval(confirm, kk);
+            b.fixed_points(fps);
             for (int i = 0; i < B; ++i) {
                 if (!confirm[size_t(i)]) continue;
                 best[size_t(i)] = rel_of(kslot(kk, i), *P[size_t(i)], o.row_rel);
@@ -505,7 +524,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
                     continue;
                 }
                 r[size_t(i)].refinements++;
-                k_last[size_t(i)] = wkkt(kslot(kk, i), w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
+                k_last[size_t(i)] = wkkt(i, w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
             }
         }
         if (any_rs) {
@@ -517,7 +536,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
             b.restart(rs_any);
             for (int i = 0; i < B; ++i)
                 if (rs_any[size_t(i)])
-                    k_last[size_t(i)] = wkkt(kslot(kk, i), w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
+                    k_last[size_t(i)] = wkkt(i, w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
         }
         if ((o.verbose && (it / o.eval_freq) % 50 == 0) || progress_due(o, t_progress)) {
             double worst = 0;

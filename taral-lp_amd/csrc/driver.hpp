@@ -52,6 +52,10 @@ struct Options {  // defaults of gpu/pdhg.cu:744-750 plus the options added here
                                  // scheme always uses the reference update
     bool reuse_norm = false;     // sessions and batches: power iteration on the first solve only
     int cpu_threads = -1;        // >= 0: the reference scheme on the CPU backend with this many threads (0: auto)
+    int restart_metric = 1;      // fast path and batches: what the restart tests measure. 1: the fixed-point residual
+                                 // ||u - T u||_P of the current point in the PDHG norm at the current weight; 0: the
+                                 // weighted KKT error of the reference (gpu/pdhg.cu:783-786). The reference scheme
+                                 // always uses the latter
 };
 
 struct Outcome {

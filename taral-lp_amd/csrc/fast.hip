@@ -114,17 +114,25 @@ struct FastEvalArgs {
     const T *Ax, *z, *z0, *x, *ATz, *x0;             // inner point and anchors
     const double *Axb, *zb, *xb, *ATzb;              // FP64 base and its products
     const double *rl, *ru, *Dr, *c, *l, *u, *Dc;     // original scaled problem, FP64
+    const T *Aw, *xp, *zh, *rl_in, *ru_in, *par;     // fixed-point residual: A w, x+, the inner row prox data
 };
 
-// KKT scalars of base + inner (the ten slots of the averaged scheme's evaluation, gpu/pdhg.cu:236-271), FP64.
-// Blocks [0, nbr) cover rows and write PR2S..DZ2; blocks [nbr, grid) cover columns and write DR2S..DX2.
+// KKT scalars of base + inner (the ten slots of the averaged scheme's evaluation, gpu/pdhg.cu:236-271), FP64,
+// and the fixed-point residual terms of the inner point. Blocks [0, nbr) cover rows and write PR2S..DZ2, then
+// ||z - z+||^2 and (z - z+)' A (x - x+), with A (x - x+) = (A x - A w) / 2 because w = 2 x+ - x; blocks
+// [nbr, grid) cover columns and write DR2S..DX2, then ||x - x+||^2 and 0. x+ is the primal step the last column
+// step took from the current point and z+ = prox(z + sigma A w) the dual step that follows it, so (x+, z+) = T(u).
 template <class T>
 __global__ __launch_bounds__(kWG) void k_eval_fast(int m, int n, int nbr, FastEvalArgs<T> a,
                                                    double* __restrict__ part) {
-    double v[5] = {0, 0, 0, 0, 0};
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
     const int b = int(blockIdx.x);
     if (b < nbr) {
+        const T sig = a.par[1];
         for (int i = b * kWG + int(threadIdx.x); i < m; i += nbr * kWG) {
+            const T zp = row_prox(a.z[i], a.Aw[i], a.zh[i], a.rl_in[i], a.ru_in[i], sig);
+            const double dzt = double(a.z[i]) - double(zp), adx = 0.5 * (double(a.Ax[i]) - double(a.Aw[i]));
+            v[5] += dzt * dzt, v[6] += dzt * adx;
             const double ax = a.Axb[i] + double(a.Ax[i]), zz = a.zb[i] + double(a.z[i]);
             const double lo = a.rl[i], up = a.ru[i], dr = a.Dr[i];
             const double r = ax < lo ? lo - ax : (ax > up ? ax - up : 0), ro = r / dr;
@@ -151,9 +159,11 @@ __global__ __launch_bounds__(kWG) void k_eval_fast(int m, int n, int nbr, FastEv
             }
             const double ro = res / a.Dc[j], dx = double(a.x[j]) - double(a.x0[j]);
             v[0] += res * res, v[1] += ro * ro, v[4] += dx * dx;
+            const double dxt = double(a.x[j]) - double(a.xp[j]);
+            v[5] += dxt * dxt;
         }
     }
-    block_store<5>(v, part);
+    block_store<7>(v, part);
 }
 
 template <class T>
@@ -181,10 +191,12 @@ public:
         x = alloc(n_), xp = alloc(n_), w = alloc(n_), x0 = alloc(n_), ATz = alloc(n_);
         z = alloc(m_), z0 = alloc(m_), Ax = alloc(m_);
         pv = alloc(n_), pw = alloc(m_), pu = alloc(n_);
+        for (auto [p, k] : {std::pair<T*, int>{xp, n_}, {w, n_}, {pw, m_}})  // read by eval before a first step
+            HIP_OK(hipMemsetAsync(p, 0, size_t(std::max(k, 1)) * sizeof(T), st_));
         par_ = alloc(2);
         kc_ = mem_.alloc<int>(1);
         nbr_ = red_blocks(m_), nbc_ = red_blocks(n_);
-        part_ = mem_.alloc<double>(size_t(std::max(1, nbr_ + nbc_)) * 5 + size_t(kMaxRedBlocks));
+        part_ = mem_.alloc<double>(size_t(std::max(1, nbr_ + nbc_)) * 7 + size_t(kMaxRedBlocks));
         out_ = mem_.alloc<double>(16);
         HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hout_), 16 * sizeof(double), hipHostMallocDefault));
         k_set_int<<<1, 1, 0, st_>>>(kc_, 0);
@@ -279,20 +291,25 @@ public:
 
     void eval(double out[NKKT]) override {
         launch_rows(A_, x, EpiStore<T>{Ax}, st_);  // A x of the inner point; A^T z was stored by the last step
+        launch_rows(A_, w, EpiStore<T>{pw}, st_);  // A w, w = 2 x+ - x: the dual half of T(u)
         if (nbr_ + nbc_ > 0) {
-            FastEvalArgs<T> a{Ax, z, z0, x, ATz, x0, Axb, zb, xb, ATzb, rl64, ru64, Dr64, c64, l64, u64, Dc64};
+            FastEvalArgs<T> a{Ax, z, z0, x, ATz, x0, Axb, zb, xb, ATzb, rl64, ru64, Dr64, c64, l64, u64, Dc64,
+                              pw, xp, zh, rl_in, ru_in, par_};
             k_eval_fast<T><<<nbr_ + nbc_, kWG, 0, st_>>>(m_, n_, nbr_, a, part_);
-            k_finish<<<1, kWG, 0, st_>>>(0, nbr_, 5, part_, out_);
-            k_finish<<<1, kWG, 0, st_>>>(nbr_, nbc_, 5, part_, out_ + 5);
-            HIP_OK(hipMemcpyAsync(hout_, out_, NKKT * sizeof(double), hipMemcpyDeviceToHost, st_));
+            k_finish<<<1, kWG, 0, st_>>>(0, nbr_, 7, part_, out_);
+            k_finish<<<1, kWG, 0, st_>>>(nbr_, nbc_, 7, part_, out_ + 7);
+            HIP_OK(hipMemcpyAsync(hout_, out_, 14 * sizeof(double), hipMemcpyDeviceToHost, st_));
             HIP_OK(hipStreamSynchronize(st_));
         } else {
-            std::fill(hout_, hout_ + NKKT, 0.0);
+            std::fill(hout_, hout_ + 14, 0.0);
         }
         HIP_OK(hipGetLastError());
-        std::copy(hout_, hout_ + NKKT, out);
+        std::copy(hout_, hout_ + 5, out), std::copy(hout_ + 7, hout_ + 12, out + 5);
+        fp_[0] = hout_[12], fp_[1] = hout_[5], fp_[2] = hout_[6];
         collect_events();
     }
+
+    void fixed_point(double out[3]) const override { std::copy(fp_, fp_ + 3, out); }
 
     void end_round() override {
         k_accumulate<T><<<grid_for(n_), kWG, 0, st_>>>(n_, xb, x);
@@ -436,6 +453,7 @@ private:
     T *pv, *pw, *pu, *par_;
     int* kc_ = nullptr;
     double *part_ = nullptr, *out_ = nullptr, *hout_ = nullptr;
+    double fp_[3] = {0, 0, 0};  // fixed-point residual terms of the last eval
     int nbr_ = 0, nbc_ = 0;
     size_t mem_used_ = 0;
     std::string name_;

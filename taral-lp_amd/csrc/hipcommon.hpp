@@ -45,6 +45,15 @@ __device__ __forceinline__ T clampv(T v, T lo, T hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Dual prox step of a recentred row (the fast path's row step before its Halpern blend): for dual z, row product
+// a, base dual h, shifted bounds [lo, hi] and step sig, z+ = v - sig * clamp((v + h) / sig, lo, hi) with
+// v = z + sig * a, written casewise (the middle case is exactly -h).
+template <class T>
+__device__ __forceinline__ T row_prox(T zi, T a, T h, T lo, T hi, T sig) {
+    const T v = zi + sig * a, q = (v + h) / sig;
+    return q < lo ? v - sig * lo : (q > hi ? v - sig * hi : -h);
+}
+
 template <class T>
 struct EpiStore {  // y = M v
     T* y;

@@ -30,7 +30,8 @@ EXPORTS = [
     "tpdhg_dev_eval", "tpdhg_dev_eval2", "tpdhg_dev_restart", "tpdhg_dev_download", "tpdhg_dev_spmv",
     "tpdhg_dev_info", "tpdhg_dev_set_profile", "tpdhg_dev_kernel_times", "tpdhg_dev_destroy",
     "tpdhg_fast_create", "tpdhg_fast_power_norm", "tpdhg_fast_set_base", "tpdhg_fast_begin_round",
-    "tpdhg_fast_set_steps", "tpdhg_fast_restart", "tpdhg_fast_iterate", "tpdhg_fast_eval", "tpdhg_fast_end_round",
+    "tpdhg_fast_set_steps", "tpdhg_fast_restart", "tpdhg_fast_iterate", "tpdhg_fast_eval", "tpdhg_fast_fixed_point",
+    "tpdhg_fast_end_round",
     "tpdhg_fast_download", "tpdhg_fast_destroy",
     "tpdhg_options_default", "tpdhg_solve", "tpdhg_session_create", "tpdhg_session_solve", "tpdhg_session_free",
     "tpdhg_session_set_vectors", "tpdhg_session_set_primal_weight",
@@ -74,7 +75,7 @@ class Options(C.Structure):
                 ("use_graph", C.c_int32), ("verbose", C.c_int32), ("profile_kernels", C.c_int32),
                 ("halpern", C.c_int32), ("reflection", C.c_double), ("primal_weight", C.c_double),
                 ("warm_fallback", C.c_int32), ("progress_interval", C.c_double), ("omega_rule", C.c_int32),
-                ("reuse_norm", C.c_int32), ("cpu_threads", C.c_int32)]
+                ("reuse_norm", C.c_int32), ("cpu_threads", C.c_int32), ("restart_metric", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -157,6 +158,7 @@ def lib():
         L.tpdhg_fast_set_steps.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.tpdhg_fast_iterate.argtypes = [C.c_void_p, C.c_int32]
         L.tpdhg_fast_eval.argtypes = [C.c_void_p, _d]
+        L.tpdhg_fast_fixed_point.argtypes = [C.c_void_p, _d]
         L.tpdhg_fast_download.argtypes = [C.c_void_p, _d, _d]
         L.tpdhg_options_default.argtypes = [C.POINTER(Options)]
         L.tpdhg_solve.argtypes = [C.c_void_p, C.POINTER(Options), _d, _d, C.POINTER(Result), _d, _d]
@@ -420,6 +422,12 @@ class FastDevice:
     def eval(self):
         out = np.zeros(10)
         _check(lib().tpdhg_fast_eval(self._h, _p(out)))
+        return out
+
+    def fixed_point(self):
+        """(||x - x+||^2, ||z - z+||^2, (z - z+)' A (x - x+)) of the inner point at the last eval()."""
+        out = np.zeros(3)
+        _check(lib().tpdhg_fast_fixed_point(self._h, _p(out)))
         return out
 
     def end_round(self):

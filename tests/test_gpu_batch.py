@@ -35,14 +35,16 @@ def _mixed():
     return lps + [packing]
 
 
-def test_batch_matches_single_solves_bit_for_bit():
+@pytest.mark.parametrize("metric", [1, 0])
+def test_batch_matches_single_solves_bit_for_bit(metric):
+    """Both restart measures (restart_metric 1: the fixed-point residual, the default; 0: the weighted KKT error)."""
     lps = _mixed()
     models = [ffi.Model.from_lp(md) for md in lps]
-    batch = ffi.Batch(models, cap=2048, **FAST)
+    batch = ffi.Batch(models, cap=2048, restart_metric=metric, **FAST)
     assert batch.info()["cap"] == 2048 and batch.info()["unique_matrices"] == len(lps)
     rb = batch.solve()
     for md, model, b in zip(lps, models, rb):
-        s = ffi.solve(model, **FAST)
+        s = ffi.solve(model, restart_metric=metric, **FAST)
         print(md.name, b["status"], b["iterations"], s["iterations"], b["objective"], s["objective"])
         assert b["status"] == s["status"] == "near_optimal"
         assert (b["iterations"], b["restarts"], b["refinements"]) == (s["iterations"], s["restarts"], s["refinements"])

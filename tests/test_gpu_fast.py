@@ -49,7 +49,23 @@ def _halpern_numpy(S, tau, sig, rho, steps, xb, zb):
         zp = np.where(q < rl_in, v - sig * rl_in, np.where(q > ru_in, v - sig * ru_in, -zb))
         x = lam * (rho * xp - (rho - 1) * x) + (1 - lam) * x0
         z = lam * (rho * zp - (rho - 1) * z) + (1 - lam) * z0
-    return xb + x, zb + z, xb + x0, zb + z0
+    return xb + x, zb + z, xb + x0, zb + z0, x, z
+
+
+def _fixed_point_numpy(S, tau, sig, xb, zb, x, z):
+    """(||x - x+||^2, ||z - z+||^2, (z - z+)' A (x - x+)) for one plain PDHG step (x+, z+) = T(x, z) of the problem
+    recentred at (xb, zb), written from the step's definition."""
+    ptr, idx, val = S.csr("A")
+    A = sp.csr_matrix((val, idx, ptr), shape=(S.view.m, S.view.n))
+    c, l, u, rl, ru = (S.vec(k) for k in ("c", "l", "u", "rl", "ru"))
+    c_in, l_in, u_in = c + A.T @ zb, l - xb, u - xb
+    rl_in, ru_in = rl - A @ xb, ru - A @ xb
+    xp = np.clip(x - tau * (c_in + A.T @ z), l_in, u_in)
+    v = z + sig * (A @ (2 * xp - x))
+    q = (v + zb) / sig
+    zp = np.where(q < rl_in, v - sig * rl_in, np.where(q > ru_in, v - sig * ru_in, -zb))
+    dx, dz = x - xp, z - zp
+    return np.array([dx @ dx, dz @ dz, dz @ (A @ dx)])
 
 
 @pytest.mark.parametrize("which", ["cfg0", "packing"])
@@ -71,7 +87,7 @@ def test_halpern_block_fp64(which, with_base, cfg0):
     fd.restart()
     fd.iterate(16)  # graph-captured block
     x, z = fd.download()
-    xw, zw, xa, za = _halpern_numpy(S, tau, sig, 1.9, 16, xb, zb)
+    xw, zw, xa, za, xi, zi = _halpern_numpy(S, tau, sig, 1.9, 16, xb, zb)
     scale = max(1.0, np.abs(xw).max(), np.abs(zw).max())
     assert np.max(np.abs(x - xw)) <= 1e-10 * scale and np.max(np.abs(z - zw)) <= 1e-10 * scale
     # FP64 evaluation of the combined point equals the reference evaluation of the same vectors
@@ -80,6 +96,35 @@ def test_halpern_block_fp64(which, with_base, cfg0):
     want = O.eval(A @ xw, zw, xw, A.T @ zw, xa, za)
     got = fd.eval()
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(want).max()))
+    # the restart measure's terms: one plain PDHG step from the inner point, from its definition
+    fp, fw = fd.fixed_point(), _fixed_point_numpy(S, tau, sig, xb, zb, xi, zi)
+    np.testing.assert_allclose(fp, fw, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(fw).max()))
+    assert fw[0] > 0 and fw[1] > 0 and fw[0] / tau + fw[1] / sig + 2 * fw[2] > 0  # the PDHG norm is positive
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_fixed_point_residual_vanishes_at_the_planted_optimum(bits, cfg0):
+    """At an optimum (x*, y*) one PDHG step moves nowhere: recentred there, the inner point is 0 and T(0) = 0 (the
+    reduced cost s >= 0 holds x at its bound where x* = 0 and vanishes elsewhere; the row residual is 0)."""
+    S = ffi.Scaled(ffi.Model.from_lp(cfg0))
+    fd = ffi.FastDevice(S, precision_bits=bits, eval_freq=16, use_graph=False, reflection=1.9)
+    norm = fd.power_norm()
+    xs = cfg0.x_star / S.vec("Dc")
+    zs = -cfg0.y_star / S.vec("Dr")  # the scaled dual in the solver's sign convention (tpdhg_solve returns y = Dr z)
+    fd.set_base(xs, zs)
+    fd.begin_round()
+    fd.set_steps(0.95 / norm, 0.95 / norm)
+    fd.restart()
+    fd.eval()
+    fp = fd.fixed_point()
+    scale = np.dot(xs, xs) + np.dot(zs, zs)
+    assert np.all(np.abs(fp) <= (1e-24 if bits == 64 else 1e-10) * scale), fp
+    # away from the optimum it does not vanish
+    fd.set_base(np.zeros(S.view.n), np.zeros(S.view.m))
+    fd.begin_round()
+    fd.restart()
+    fd.eval()
+    assert fd.fixed_point()[1] > 1e-6 * scale
 
 
 @pytest.mark.parametrize("tol", [1e-4, 1e-6])
