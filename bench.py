@@ -140,6 +140,8 @@ def solver_opts(a, mode_name, **extra):
              verbose=int(a.verbose), progress_interval=a.progress, **MODES[mode_name])
     if a.omega_rule is not None:
         o["omega_rule"] = a.omega_rule
+    if a.restart_metric is not None:
+        o["restart_metric"] = a.restart_metric
     if a.reflection is not None and o.get("halpern"):
         o["reflection"] = a.reflection
     o.update(extra)
@@ -413,6 +415,8 @@ def run_scenarios(a, ffi, gen, desc, fam, budget):
     th_tr, th_te = fam.thetas(a.train_scenarios, seed=1), fam.thetas(a.test_scenarios, seed=2)
     bs = min(a.scenario_batch, a.test_scenarios)
     extra = {} if a.omega_rule is None else {"omega_rule": a.omega_rule}
+    if a.restart_metric is not None:
+        extra["restart_metric"] = a.restart_metric
     fp64 = None
     if not a.no_fp64_compare and a.fp64_sample > 0:
         # plain FP64 cold starts of held-out scenarios on one session of the base model (its norm reused, as the
@@ -533,6 +537,9 @@ def main():
     ap.add_argument("--omega-rule", type=int, default=None,
                     help="fast path: primal-weight guard (0 reference update, 1 bounded, 2 balance, 3 both; "
                          "default: the library's, 3)")
+    ap.add_argument("--restart-metric", type=int, default=None,
+                    help="fast path: what the restart tests measure (1 the fixed-point residual, 0 the weighted KKT "
+                         "error; default: the library's, 1)")
     ap.add_argument("--scenario-m", type=int, default=5000,
                     help="scenario workloads: rows of the base LP (n = 2m); the configuration's 50,000 needs ~1.5M "
                          "iterations per scenario")

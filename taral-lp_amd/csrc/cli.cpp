@@ -168,6 +168,7 @@ int main(int argc, char** argv) {
         else if (a == "--refine") o.refine = true, o.halpern = true;
         else if (a == "--halpern") o.halpern = true;
         else if (a == "--reflection") o.rho = std::atof(next());
+        else if (a == "--restart-metric") o.restart_metric = std::atoi(next());
         else if (a == "--fast") o.halpern = true, o.refine = true, o.precision_bits = 32;
         else if (a == "--max-refinements") o.max_refinements = std::atoi(next());
         else if (a == "--profile-kernels") o.profile_kernels = true;
@@ -182,7 +183,7 @@ int main(int argc, char** argv) {
         std::fprintf(stderr,
                      "usage: pdhg MODEL.mps --device gpu|cpu [--threads N] [--tol 1e-4] [--time-limit S] "
                      "[--max-iter N] [--json OUT.json] [--sol OUT.sol] [--dual OUT.dual] [--fp32] [--eval-freq 64] "
-                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--refine] "
+                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--restart-metric 1] [--refine] "
                      "[--max-refinements 8] "
                      "[--profile-kernels] [--warm-sol X.sol] [--warm-dual X.dual] [--recheck]\n"
                      "       pdhg --batch LIST.txt [--json OUT.jsonl] [solver flags]\n");
