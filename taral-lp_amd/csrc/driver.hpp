@@ -76,7 +76,8 @@ Outcome run_pdhg(Device& dev, const Scaled& P, const Options& o, double time_bud
                  const double* z0 = nullptr, double omega0 = 0, double norm_a = 0);
 
 // The fast path (Options::halpern): reflected Halpern PDHG iterations on the FastDevice with the restart tests and
-// primal-weight update of run_pdhg (the candidate is the current point). With o.refine (FP64 iterative
+// primal-weight update of run_pdhg (the candidate is the current point), the tests applied to o.restart_metric's
+// measure (by default the fixed-point residual of one PDHG step in its own norm). With o.refine (FP64 iterative
 // refinement) every restart also recentres: the base point moves to the restart point in FP64 and the next inner
 // problem is rebuilt with exact FP64 products, so the FP32 inner iterates only ever span one restart cycle (at most
 // o.max_refinements recentrings). A point that meets the tolerance is confirmed by an FP64 evaluation after a
