@@ -118,6 +118,24 @@ def test_options_carry_warm_start_fields():
     assert ffi.Result().warm_start_used == 0
 
 
+def test_restart_metric_option():
+    """The fast path restarts on the fixed-point residual by default; only 0 (weighted KKT) and 1 are accepted,
+    refused on the host before any device work."""
+    from taral_pdhg import gen
+    assert ffi.default_options().restart_metric == 1 and ffi.default_options().omega_rule == 3
+    md = ffi.Model.from_lp(gen.planted_lp(20, 40, 3, seed=0))
+    for bad in (-1, 2):
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.solve(md, halpern=1, restart_metric=bad)
+        assert e.value.code == ffi.ERR_ARG and "restart_metric" in str(e.value)
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.Batch([md], restart_metric=bad)
+        assert e.value.code == ffi.ERR_ARG and "restart_metric" in str(e.value)
+    with pytest.raises(ffi.TpdhgError) as e:  # a valid setting reaches the device check
+        ffi.solve(md, halpern=1, restart_metric=0)
+    assert e.value.code == ffi.ERR_NO_DEVICE
+
+
 def test_batch_create_validates_before_the_device():
     from taral_pdhg import gen
     md = ffi.Model.from_lp(gen.planted_lp(20, 40, 3, seed=0))
