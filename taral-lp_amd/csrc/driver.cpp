@@ -186,7 +186,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
     // evaluation, ||u - T u||_P = sqrt(||dx||^2 / tau + ||dz||^2 / sigma + 2 dz' A dx) with tau = eta / w and
     // sigma = eta w (the norm in which a PDHG step is nonexpansive; Halpern iterations drive it down at a known
     // rate between restarts), or the reference's weighted KKT error.
-    auto wkkt = [&](const double* k) {  // the restart measure (named for the reference's weighted KKT error)
+    auto measure = [&](const double* k) {  // what the restart tests compare
         if (o.restart_metric == 1) {
             double f[3];
             b.fixed_point(f);
@@ -203,7 +203,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
     b.restart();
     b.eval(kk);
     Rel best = rel(kk);
-    double k_last = wkkt(kk), k_prev = kInf;
+    double k_last = measure(kk), k_prev = kInf;
     // Each recentring folds the inner point into the FP64 base and rebuilds the inner problem with exact FP64
     // products; the next inner solve then only spans one restart cycle, so its FP32 rounding stays relative to
     // that cycle's (shrinking) distance instead of to the distance from the start.
@@ -239,7 +239,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
                 break;
             }
             r.refinements++;  // not yet: carry on from the recentred point
-            k_last = wkkt(kk), k_prev = kInf, n_since = 0;
+            k_last = measure(kk), k_prev = kInf, n_since = 0;
             continue;
         }
         if (since(t0) > time_budget) {
@@ -249,7 +249,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
         if (it >= o.max_iter) break;
         // The restart tests of gpu/pdhg.cu:827-842 (decay to 0.2; to 0.8 without progress; 36% of the iterations)
         // on the restart measure of the current point, the candidate.
-        const double kc = wkkt(kk);
+        const double kc = measure(kk);
         const bool restart =
             double(n_since) >= 0.36 * double(it) || kc <= 0.2 * k_last || (kc <= 0.8 * k_last && kc > k_prev);
         k_prev = kc;
@@ -263,7 +263,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
                 b.set_steps(eta / w, eta * w);
                 b.restart();
             }
-            k_last = wkkt(kk), k_prev = kInf, n_since = 0, r.restarts++;
+            k_last = measure(kk), k_prev = kInf, n_since = 0, r.restarts++;
         }
     }
     r.iterations = it;
@@ -438,7 +438,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
     std::vector<uint8_t> all(static_cast<size_t>(B), 1), active(static_cast<size_t>(B), 0);
     std::vector<double> kk(size_t(B) * NKKT, 0.0), k_last(static_cast<size_t>(B)), k_prev(static_cast<size_t>(B), kInf);
     std::vector<double> fps;  // fixed-point residual terms of the last evaluation, 3 per instance
-    auto wkkt = [&](int i, double wi) {  // run_fast's restart measure for instance i at weight wi
+    auto measure = [&](int i, double wi) {  // run_fast's restart measure for instance i at weight wi
         if (o.restart_metric == 1) {
             const double ei = eta[size_t(i)];
             const double* f = fps.data() + 3 * size_t(i);
@@ -458,7 +458,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
     int live = 0;
     for (int i = 0; i < B; ++i) {
         best[size_t(i)] = rel_of(kslot(kk, i), *P[size_t(i)], o.row_rel);
-        k_last[size_t(i)] = wkkt(i, w[size_t(i)]);
+        k_last[size_t(i)] = measure(i, w[size_t(i)]);
         if (best[size_t(i)].worst() <= o.tol) r[size_t(i)].status = "near_optimal";
         else active[size_t(i)] = 1, ++live;
     }
@@ -498,7 +498,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
                 active[size_t(i)] = 0, --live;
                 continue;
             }
-            const double kc = wkkt(i, w[size_t(i)]);
+            const double kc = measure(i, w[size_t(i)]);
             const bool restart = double(n_since[size_t(i)]) >= 0.36 * double(it) || kc <= 0.2 * k_last[size_t(i)] ||
                                  (kc <= 0.8 * k_last[size_t(i)] && kc > k_prev[size_t(i)]);
             k_prev[size_t(i)] = kc;
@@ -525,7 +525,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
                     continue;
                 }
                 r[size_t(i)].refinements++;
-                k_last[size_t(i)] = wkkt(i, w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
+                k_last[size_t(i)] = measure(i, w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
             }
         }
         if (any_rs) {
@@ -537,7 +537,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
             b.restart(rs_any);
             for (int i = 0; i < B; ++i)
                 if (rs_any[size_t(i)])
-                    k_last[size_t(i)] = wkkt(i, w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
+                    k_last[size_t(i)] = measure(i, w[size_t(i)]), k_prev[size_t(i)] = kInf, n_since[size_t(i)] = 0;
         }
         if ((o.verbose && (it / o.eval_freq) % 50 == 0) || progress_due(o, t_progress)) {
             double worst = 0;
