@@ -30,7 +30,7 @@ bool progress_due(const Options& o, Clock::time_point& last) {
 // Primal weight after a restart. Rule 0 is the reference's smoothed update exp(0.5 ln(dz/dx) + 0.5 ln w)
 // (gpu/pdhg.cu:839-840), skipped when either distance is below 1e-10. The fast path's optional guards:
 //   1  keep the weight within two decades of the solve's starting weight w_start;
-//   2  residual balancing: when the relative primal residual exceeds ten times the larger of the dual residual
+//   2  residual balancing: when the relative primal residual exceeds three times the larger of the dual residual
 //      and the gap, the weight may not fall (a small weight makes the primal step large and the dual step small,
 //      which leaves the primal residual behind); in the opposite case it may not rise;
 //   3  both.
@@ -38,8 +38,8 @@ double next_omega(double w, double dx, double dz, double p, double dg, double w_
     double nw = w;
     if (dx > 1e-10 && dz > 1e-10) nw = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w));
     if (rule == 2 || rule == 3) {
-        if (p > 10 * dg) nw = std::max(nw, w);
-        else if (dg > 10 * p) nw = std::min(nw, w);
+        if (p > 3 * dg) nw = std::max(nw, w);
+        else if (dg > 3 * p) nw = std::min(nw, w);
     }
     if (rule == 1 || rule == 3) nw = std::min(std::max(nw, w_start * 1e-2), w_start * 1e2);
     return nw;
