@@ -61,11 +61,18 @@ def test_planted_fp64_matches_oracle(cfg0, golden_dir, tol):
 
 
 def test_planted_fp32(cfg0):
+    """The reference's --fp32 (FP32 iterates and products, FP64 reductions) against the oracle's FP32 mode: the
+    summation orders differ (row blocks here, CSR order there), so the iterates differ in their last FP32 bits, but
+    both are deterministic and the restart decisions are not near ties on this LP: the same iterations and
+    restarts, objectives within the tolerance of each other."""
     print("cfg0 fp32 tol=1e-4", flush=True)
     r = ffi.solve(ffi.Model.from_lp(cfg0), tol=1e-4, precision_bits=32)
     o = oracle.OracleLP(cfg0).solve(tol=1e-4, fp32=True)
-    assert r["status"] == "near_optimal"
-    assert abs(r["iterations"] - o["iterations"]) <= 4 * 64
+    print(f"  gpu it={r['iterations']} restarts={r['restarts']} obj={r['objective']:.10g}; oracle it={o['iterations']} "
+          f"restarts={o['restarts']} obj={o['objective']:.10g}", flush=True)
+    assert r["status"] == "near_optimal" == o["status"]
+    assert (r["iterations"], r["restarts"]) == (o["iterations"], o["restarts"])
+    assert abs(r["objective"] - o["objective"]) / (1 + abs(o["objective"])) <= 1e-4
     assert abs(r["objective"] - cfg0.planted_objective) / (1 + abs(cfg0.planted_objective)) <= 3e-4
 
 
