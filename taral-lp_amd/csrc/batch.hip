@@ -775,7 +775,7 @@ public:
         if (cs.second)
             k_beval_cols<T><<<cs.second, kWG, 0, st_>>>(cs.first, dflags_, a, part_ + size_t(nrseg_) * kEvalK);
         k_bfinish<<<B_, kWG, 0, st_>>>(d_rfirst_, d_rcount_, dflags_, kEvalK, part_, 2 * kEvalK, 0, out_);
-        k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst5_, d_ccount_, dflags_, kEvalK, part_, 2 * kEvalK, kEvalK, out_);
+        k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst_after_rows_, d_ccount_, dflags_, kEvalK, part_, 2 * kEvalK, kEvalK, out_);
         HIP_OK(hipMemcpyAsync(hout_, out_, size_t(B_) * 2 * kEvalK * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIP_OK(hipStreamSynchronize(st_));
         HIP_OK(hipGetLastError());
@@ -997,9 +997,9 @@ private:
         d_rlive_ = mem_.alloc<int>(std::max<size_t>(rtab_.size(), 4)), d_clive_ = mem_.alloc<int>(std::max<size_t>(ctab_.size(), 4));
         d_rfirst_ = up_int(rfirst_), d_rcount_ = up_int(rcount_), d_cfirst_ = up_int(cfirst_), d_ccount_ = up_int(ccount_);
         {
-            std::vector<int32_t> c5(cfirst_);
-            for (int32_t& v : c5) v += nrseg_;  // column partials follow the row partials in part_
-            d_cfirst5_ = up_int(c5);
+            std::vector<int32_t> cf(cfirst_);
+            for (int32_t& v : cf) v += nrseg_;  // column partials follow the row partials in part_
+            d_cfirst_after_rows_ = up_int(cf);
         }
         c64_ = cat64(probs, 0, &Scaled::c), l64_ = cat64(probs, 0, &Scaled::l), u64_ = cat64(probs, 0, &Scaled::u);
         Dc64_ = cat64(probs, 0, &Scaled::Dc);
@@ -1488,7 +1488,7 @@ private:
     int *d_rtab_ = nullptr, *d_ctab_ = nullptr, *d_rfirst_ = nullptr, *d_rcount_ = nullptr;
     int *d_rlive_ = nullptr, *d_clive_ = nullptr;
     int nrlive_ = 0, nclive_ = 0;
-    int *d_cfirst_ = nullptr, *d_cfirst5_ = nullptr, *d_ccount_ = nullptr;
+    int *d_cfirst_ = nullptr, *d_cfirst_after_rows_ = nullptr, *d_ccount_ = nullptr;
     double *c64_ = nullptr, *l64_ = nullptr, *u64_ = nullptr, *Dc64_ = nullptr, *rl64_ = nullptr, *ru64_ = nullptr,
            *Dr64_ = nullptr;
     double *xb_ = nullptr, *zb_ = nullptr, *Axb_ = nullptr, *ATzb_ = nullptr, *buf64_ = nullptr, *fac_ = nullptr;
