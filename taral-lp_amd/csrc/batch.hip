@@ -425,6 +425,17 @@ __global__ __launch_bounds__(kWG) void k_bsumsq(Seg S, const uint8_t* __restrict
     block_store<1>(v, part, slot_);
 }
 
+// fac[i] = 1 / sqrt(s[i]) for the flagged instances with s[i] > 0 (the device-side normalisation of a power step;
+// the same arithmetic as k_scale_by_sum), 1 otherwise.
+__global__ void k_inv_sqrt_pos(int B, const uint8_t* __restrict__ on, const double* __restrict__ s,
+                               double* __restrict__ fac) {
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < B; i += int(gridDim.x * blockDim.x))
+        if (on[i]) {
+            const double sv = s[i];
+            fac[i] = sv > 0 ? 1 / sqrt(sv) : 1.0;
+        }
+}
+
 __global__ void k_bset_counters(int B, const uint8_t* __restrict__ on, int* __restrict__ kc, int add, int reset) {
     for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < B; i += int(gridDim.x * blockDim.x))
         if (on[i]) kc[i] = reset ? 0 : kc[i] + add;
@@ -624,7 +635,8 @@ public:
 
     std::vector<double> power_norm(int iters) override {
         tuned_stage();
-        std::vector<double> est(static_cast<size_t>(B_), 0.0), prev(static_cast<size_t>(B_), 0.0), fac(static_cast<size_t>(B_), 1.0);
+        static_assert(kPowerGroup <= 2 * kEvalK, "a power-iteration group's sums of squares go to out_");
+        std::vector<double> est(static_cast<size_t>(B_), 0.0), prev(static_cast<size_t>(B_), 0.0);
         {
             std::vector<T> hv(static_cast<size_t>(ntot_));
             for (int i = 0; i < B_; ++i) {
@@ -640,39 +652,42 @@ public:
             HIP_OK(hipMemcpyAsync(pv_, hv.data(), hv.size() * sizeof(T), hipMemcpyHostToDevice, st_));
             HIP_OK(hipStreamSynchronize(st_));
         }
+        // Each instance follows the single solve's power iteration (FastDev::power_norm: steps normalised on the
+        // device, kPowerGroup steps between host reads, the stopping rule of gpu/pdhg.cu:417-436); an instance that
+        // stops inside a group keeps iterating to the group's end, its later steps discarded.
         std::vector<uint8_t> on(static_cast<size_t>(B_), 1);
         T *pv = pv_, *pu = pu_;
-        for (int it = 0; it < iters; ++it) {
+        bool any = true;
+        for (int it0 = 0; it0 < iters && any; it0 += kPowerGroup) {
+            const int g = std::min(kPowerGroup, iters - it0);
             upload_flags(on);
-            spmv_rows(A_, pv, BStore<T>{pw_, 0}, dflags_);
-            spmv_rows(AT_, pw_, BStore<T>{pu, 0}, dflags_);
-            k_bsumsq<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, pu, part_);
-            k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst_, d_ccount_, dflags_, 1, part_, 1, 0, out_);
-            HIP_OK(hipMemcpyAsync(hout_, out_, size_t(B_) * sizeof(double), hipMemcpyDeviceToHost, st_));
-            HIP_OK(hipStreamSynchronize(st_));
-            bool any = false;
-            for (int i = 0; i < B_; ++i) {
-                if (!on[size_t(i)]) continue;
-                const double s = hout_[i];
-                prev[size_t(i)] = est[size_t(i)];
-                if (s <= 0) {
-                    est[size_t(i)] = 0, on[size_t(i)] = 0;
-                    continue;
-                }
-                est[size_t(i)] = std::sqrt(std::sqrt(s));
-                fac[size_t(i)] = 1 / std::sqrt(s);
+            for (int q = 0; q < g; ++q) {
+                double* sq = out_ + size_t(q) * size_t(B_);
+                spmv_rows(A_, pv, BStore<T>{pw_, 0}, dflags_);
+                spmv_rows(AT_, pw_, BStore<T>{pu, 0}, dflags_);
+                k_bsumsq<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, pu, part_);
+                k_bfinish<<<B_, kWG, 0, st_>>>(d_cfirst_, d_ccount_, dflags_, 1, part_, 1, 0, sq);
+                k_inv_sqrt_pos<<<(B_ + kWG - 1) / kWG, kWG, 0, st_>>>(B_, dflags_, sq, fac_);
+                k_bscale<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, pu, fac_);
+                std::swap(pv, pu);
             }
-            HIP_OK(hipMemcpyAsync(fac_, fac.data(), size_t(B_) * sizeof(double), hipMemcpyHostToDevice, st_));
-            upload_flags(on);
-            k_bscale<T><<<ncseg_, kWG, 0, st_>>>(cseg(), dflags_, pu, fac_);
+            HIP_OK(hipMemcpyAsync(hout_, out_, size_t(g) * size_t(B_) * sizeof(double), hipMemcpyDeviceToHost, st_));
             HIP_OK(hipStreamSynchronize(st_));
-            std::swap(pv, pu);
+            any = false;
             for (int i = 0; i < B_; ++i) {
-                if (!on[size_t(i)]) continue;
-                if (it >= 10 && std::fabs(est[size_t(i)] - prev[size_t(i)]) <= 1e-7 * est[size_t(i)]) on[size_t(i)] = 0;
+                for (int q = 0; q < g && on[size_t(i)]; ++q) {
+                    const double s = hout_[size_t(q) * size_t(B_) + size_t(i)];
+                    prev[size_t(i)] = est[size_t(i)];
+                    if (s <= 0) {
+                        est[size_t(i)] = 0, on[size_t(i)] = 0;
+                        break;
+                    }
+                    est[size_t(i)] = std::sqrt(std::sqrt(s));
+                    if (it0 + q >= 10 && std::fabs(est[size_t(i)] - prev[size_t(i)]) <= 1e-7 * est[size_t(i)])
+                        on[size_t(i)] = 0;
+                }
                 any = any || on[size_t(i)];
             }
-            if (!any) break;
         }
         HIP_OK(hipGetLastError());
         return est;

@@ -197,6 +197,7 @@ public:
         kc_ = mem_.alloc<int>(1);
         nbr_ = red_blocks(m_), nbc_ = red_blocks(n_);
         part_ = mem_.alloc<double>(size_t(std::max(1, nbr_ + nbc_)) * 7 + size_t(kMaxRedBlocks));
+        static_assert(kPowerGroup <= 16, "a power-iteration group's sums of squares go to out_");
         out_ = mem_.alloc<double>(16);
         HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hout_), 16 * sizeof(double), hipHostMallocDefault));
         k_set_int<<<1, 1, 0, st_>>>(kc_, 0);
@@ -211,7 +212,10 @@ public:
         if (st_) (void)hipStreamDestroy(st_);
     }
 
-    double power_norm(int iters) override {  // same iteration and stopping rule as gpu/pdhg.cu:417-436
+    // The iteration and stopping rule of gpu/pdhg.cu:417-436, with each step normalised on the device so that
+    // kPowerGroup steps run between two host reads of their sums of squares; the estimate is the one of the step
+    // the rule stops at (the group's later steps are discarded).
+    double power_norm(int iters) override {
         std::vector<double> h(static_cast<size_t>(n_));
         double nn = 0;
         for (int j = 0; j < n_; ++j) {
@@ -221,16 +225,27 @@ public:
         std::vector<T> ht(static_cast<size_t>(n_));
         for (int j = 0; j < n_; ++j) ht[size_t(j)] = T(h[size_t(j)] / std::sqrt(nn));
         h2d(pv, ht.data(), size_t(n_));
+        const int b = red_blocks(n_);
+        if (!b) return 0;
         double est = 0;
-        for (int it = 0; it < iters; ++it) {
-            launch_rows(A_, pv, EpiStore<T>{pw}, st_);
-            launch_rows(AT_, pw, EpiStore<T>{pu}, st_);
-            const double s = sumsq(n_, pu), prev = est;
-            if (s <= 0) return 0;
-            est = std::sqrt(std::sqrt(s));
-            k_scale<T><<<grid_for(n_), kWG, 0, st_>>>(n_, pu, 1 / std::sqrt(s));
-            std::swap(pv, pu);
-            if (it >= 10 && std::fabs(est - prev) <= 1e-7 * est) break;
+        for (int it0 = 0; it0 < iters; it0 += kPowerGroup) {
+            const int g = std::min(kPowerGroup, iters - it0);
+            for (int q = 0; q < g; ++q) {
+                launch_rows(A_, pv, EpiStore<T>{pw}, st_);
+                launch_rows(AT_, pw, EpiStore<T>{pu}, st_);
+                k_sumsq<T><<<b, kWG, 0, st_>>>(n_, pu, part_);
+                k_finish<<<1, kWG, 0, st_>>>(0, b, 1, part_, out_ + q);
+                k_scale_by_sum<T><<<grid_for(n_), kWG, 0, st_>>>(n_, pu, out_ + q);
+                std::swap(pv, pu);
+            }
+            HIP_OK(hipMemcpyAsync(hout_, out_, size_t(g) * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+            for (int q = 0; q < g; ++q) {
+                const double s = hout_[q], prev = est;
+                if (s <= 0) return 0;
+                est = std::sqrt(std::sqrt(s));
+                if (it0 + q >= 10 && std::fabs(est - prev) <= 1e-7 * est) return est;
+            }
         }
         return est;
     }

@@ -398,6 +398,20 @@ __global__ __launch_bounds__(kWG) void k_scale(int n, T* __restrict__ x, double 
         x[j] = T(x[j] * f);
 }
 
+// x *= 1 / sqrt(s[0]) when s[0] > 0 (a power-iteration step normalised on the device: s is the step's sum of
+// squares, still on the device).
+template <class T>
+__global__ __launch_bounds__(kWG) void k_scale_by_sum(int n, T* __restrict__ x, const double* __restrict__ s) {
+    const double sv = *s;
+    if (!(sv > 0)) return;
+    const double f = 1 / sqrt(sv);
+    for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))
+        x[j] = T(x[j] * f);
+}
+
+// Power-iteration steps run between two reads of their sums of squares by the host.
+constexpr int kPowerGroup = 14;
+
 template <class T>
 __global__ __launch_bounds__(kWG) void k_to_f64(int n, double f, const T* __restrict__ src, double* __restrict__ dst) {
     for (int j = int(blockIdx.x * blockDim.x + threadIdx.x); j < n; j += int(gridDim.x * blockDim.x))

@@ -12,6 +12,7 @@ import os
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg  # noqa: F401  (sp.linalg.svds)
 
 from kkt_check import rel_kkt
 from oracle import oracle
@@ -100,6 +101,28 @@ def test_halpern_block_fp64(which, with_base, cfg0):
     fp, fw = fd.fixed_point(), _fixed_point_numpy(S, tau, sig, xb, zb, xi, zi)
     np.testing.assert_allclose(fp, fw, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(fw).max()))
     assert fw[0] > 0 and fw[1] > 0 and fw[0] / tau + fw[1] / sig + 2 * fw[2] > 0  # the PDHG norm is positive
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+@pytest.mark.parametrize("iters", [5, 23, 200])
+def test_power_norm_matches_the_step_by_step_loop(bits, iters, cfg0):
+    """The fast path's power iteration normalises each step on the device and reads the sums of squares back a
+    group of steps at a time; its estimate must be the one the reference's step-by-step loop (gpu/pdhg.cu:417-436,
+    restated in the oracle) reaches after the same steps: 5 (inside the first group), 23 (across a group boundary)
+    and 200 (the solver's count, with its early stop). Power iteration approaches ||A_s|| from below."""
+    S = ffi.Scaled(ffi.Model.from_lp(cfg0))
+    O = oracle.OracleLP(cfg0)
+    fd = ffi.FastDevice(S, precision_bits=bits, eval_freq=16, use_graph=False, reflection=1.9)
+    got = fd.power_norm(iters)
+    want = O.power_norm(fp32=bits == 32, iters=iters)
+    ptr, idx, val = S.csr("A")
+    smax = sp.linalg.svds(sp.csr_matrix((val, idx, ptr), shape=(S.view.m, S.view.n)), k=1,
+                          return_singular_vectors=False)[0]
+    print(f"power norm bits={bits} iters={iters}: {got!r} oracle {want!r} svd {smax!r}", flush=True)
+    assert got == pytest.approx(want, rel=1e-9 if bits == 64 else 1e-6)
+    assert got <= smax * (1 + 1e-6)
+    if iters == 200:  # 0.5832 against 0.5850 here (clustered top singular values); 0.95 / estimate stays a safe step
+        assert got >= 0.99 * smax
 
 
 @pytest.mark.parametrize("bits", [64, 32])
