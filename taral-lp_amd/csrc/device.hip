@@ -165,6 +165,7 @@ public:
         pv = zeros(n), pw = zeros(m), pu = zeros(n);
         nbr_ = red_blocks(m), nbc_ = red_blocks(n);
         part_ = mem_.alloc<double>(size_t(std::max(1, nbr_ + nbc_)) * 10 + size_t(kMaxRedBlocks));
+        static_assert(kPowerGroup <= 32, "a power-iteration group's sums of squares go to out_");
         out_ = mem_.alloc<double>(32);
         f64buf_ = mem_.alloc<double>(size_t(std::max(m, n)));
         HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hout_), 32 * sizeof(double), hipHostMallocDefault));
@@ -189,16 +190,30 @@ public:
         std::vector<T> ht(static_cast<size_t>(n_));
         for (int j = 0; j < n_; ++j) ht[size_t(j)] = T(h[size_t(j)] / std::sqrt(nn));
         h2d(pv, ht.data(), size_t(n_));
+        // The loop of gpu/pdhg.cu:417-436 with each step normalised on the device (x *= 1 / sqrt(s), s still on the
+        // device), so kPowerGroup steps run between two host reads of their sums of squares; the estimate is the
+        // one of the step the stopping rule picks, the group's later steps discarded.
+        const int b = red_blocks(n_);
+        if (!b) return 0;
         double est = 0;
-        for (int it = 0; it < iters; ++it) {
-            spmv_dev(A_, pv, pw);
-            spmv_dev(AT_, pw, pu);
-            const double s = sumsq(n_, pu), prev = est;
-            if (s <= 0) return 0;
-            est = std::sqrt(std::sqrt(s));
-            k_scale<T><<<grid_for(n_), kWG, 0, st_>>>(n_, pu, 1 / std::sqrt(s));
-            std::swap(pv, pu);
-            if (it >= 10 && std::fabs(est - prev) <= 1e-7 * est) break;
+        for (int it0 = 0; it0 < iters; it0 += kPowerGroup) {
+            const int g = std::min(kPowerGroup, iters - it0);
+            for (int q = 0; q < g; ++q) {
+                spmv_dev(A_, pv, pw);
+                spmv_dev(AT_, pw, pu);
+                k_sumsq<T><<<b, kWG, 0, st_>>>(n_, pu, part_);
+                k_finish<<<1, kWG, 0, st_>>>(0, b, 1, part_, out_ + q);
+                k_scale_by_sum<T><<<grid_for(n_), kWG, 0, st_>>>(n_, pu, out_ + q);
+                std::swap(pv, pu);
+            }
+            HIP_OK(hipMemcpyAsync(hout_, out_, size_t(g) * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIP_OK(hipStreamSynchronize(st_));
+            for (int q = 0; q < g; ++q) {
+                const double s = hout_[q], prev = est;
+                if (s <= 0) return 0;
+                est = std::sqrt(std::sqrt(s));
+                if (it0 + q >= 10 && std::fabs(est - prev) <= 1e-7 * est) return est;
+            }
         }
         return est;
     }
@@ -350,15 +365,6 @@ private:
         HIP_OK(hipStreamSynchronize(st_));
     }
     void spmv_dev(const DevMat<T>& a, const T* v, T* y) { launch_rows(a, v, EpiStore<T>{y}, st_); }
-    double sumsq(int k, const T* v) {
-        const int b = red_blocks(k);
-        if (!b) return 0;
-        k_sumsq<T><<<b, kWG, 0, st_>>>(k, v, part_);
-        k_finish<<<1, kWG, 0, st_>>>(0, b, 1, part_, out_);
-        HIP_OK(hipMemcpyAsync(hout_, out_, sizeof(double), hipMemcpyDeviceToHost, st_));
-        HIP_OK(hipStreamSynchronize(st_));
-        return hout_[0];
-    }
     void products() {  // Ax = A x, ATz = A^T z exactly
         spmv_dev(A_, x, Ax);
         spmv_dev(AT_, z, ATz);

@@ -399,15 +399,6 @@ private:
         HIP_OK(hipMemcpyAsync(dst, src, k * sizeof(T), hipMemcpyHostToDevice, st_));
         HIP_OK(hipStreamSynchronize(st_));
     }
-    double sumsq(int k, const T* v) {
-        const int b = red_blocks(k);
-        if (!b) return 0;
-        k_sumsq<T><<<b, kWG, 0, st_>>>(k, v, part_);
-        k_finish<<<1, kWG, 0, st_>>>(0, b, 1, part_, out_);
-        HIP_OK(hipMemcpyAsync(hout_, out_, sizeof(double), hipMemcpyDeviceToHost, st_));
-        HIP_OK(hipStreamSynchronize(st_));
-        return hout_[0];
-    }
     EpiHCol<T> col_epi(int s, int mode, bool store) {
         return EpiHCol<T>{x, xp, w, store ? ATz : nullptr, x0, c_in, l_in, u_in, par_, kc_, s, mode,
                           T(o_.rho), T(0), T(0)};
