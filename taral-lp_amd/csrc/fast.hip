@@ -389,6 +389,7 @@ private:
             d.nch = a.nch, d.nlr = a.nlr, d.chunk = a.chunk, d.lrow = a.lrow;
             d.lpart = reinterpret_cast<double*>(a.lpart);
             d.sorted = a.sorted, d.cbits = a.cbits, d.sb = a.sb, d.sstart = a.sstart, d.nslab = a.nslab;
+            d.bk = a.bk, d.roff = a.roff;
             return d;
         } else {
             return share_structure<double>(mem_, st_, a, L, h.val.data());

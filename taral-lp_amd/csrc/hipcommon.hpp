@@ -152,12 +152,11 @@ __global__ __launch_bounds__(kWG) void k_long_rows(int nlr, const int* __restric
     }
 }
 
-// Row sums of a staged block (products of rows [r0, r1) in CSR order at sh[ptr[r] - k0 ...]) by a workgroup of
-// WG threads: blocks of at most WG rows give each row a power-of-two lane group (xor shuffles inside a wave),
-// larger blocks one thread per row.
-template <class T, int WG = kWG, class Out>
-__device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int r0, int r1, int k0, const T* sh,
-                                               Out&& out) {
+// Row sums of a staged block (products of rows [r0, r1) in CSR order at sh[off(r) ...], off(r1) = the block's
+// length) by a workgroup of WG threads: blocks of at most WG rows give each row a power-of-two lane group (xor
+// shuffles inside a wave), larger blocks one thread per row.
+template <class T, int WG = kWG, class Off, class Out>
+__device__ __forceinline__ void block_row_sums_at(Off&& off, int r0, int r1, const T* sh, Out&& out) {
     const int tid = int(threadIdx.x), nr = r1 - r0;
     if (nr <= WG) {
         int lg = 6;
@@ -165,7 +164,7 @@ __device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int 
         const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
         T s = 0;
         if (row < nr) {
-            const int a = ptr[r0 + row] - k0, e = ptr[r0 + row + 1] - k0;
+            const int a = off(r0 + row), e = off(r0 + row + 1);
             for (int k = a + lane; k < e; k += lpr) s += sh[k];
         }
         for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
@@ -177,7 +176,7 @@ __device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int 
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int r = min(rb + q * WG + tid, r1 - 1);
-            a[q] = ptr[r] - k0, e[q] = ptr[r + 1] - k0;
+            a[q] = off(r), e[q] = off(r + 1);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -189,6 +188,13 @@ __device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int 
             }
         }
     }
+}
+
+// The same with the block's rows located by CSR row pointers (row r's products at sh[ptr[r] - k0 ...]).
+template <class T, int WG = kWG, class Out>
+__device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int r0, int r1, int k0, const T* sh,
+                                               Out&& out) {
+    block_row_sums_at<T, WG>([&](int r) { return ptr[r] - k0; }, r0, r1, sh, out);
 }
 
 // Pairwise sum of N values (N a power of two): ((r0 + r1) + (r2 + r3)) + ...
@@ -244,11 +250,14 @@ __global__ __launch_bounds__(WG) void k_sorted(int nblk, const int* __restrict__
 
 // Sorted slab layout: workgroup b takes row block b / nslab of slab b % nslab (columns local to the slab,
 // gathered from v + sstart[slab]) and writes the slab's partial row sums part[slab * rows + row]; same staging as
-// k_sorted, with WG threads.
+// k_sorted, with WG threads. A block's nonzeros are [bk[b], bk[b + 1]) and its rows are located by their 16-bit
+// offsets inside the block (roff), half the bytes of per-slab int32 row pointers.
 template <class T, int SB, int WG>
 __global__ __launch_bounds__(WG) void k_sorted_slab(int rows, int nslab, const int* __restrict__ snblk,
                                                     const int* __restrict__ sboff, const int* __restrict__ blk_all,
-                                                    const int* __restrict__ ptr_all, const uint32_t* __restrict__ pk,
+                                                    const int* __restrict__ bk_all,
+                                                    const uint16_t* __restrict__ roff_all,
+                                                    const uint32_t* __restrict__ pk,
                                                     const T* __restrict__ val, const T* __restrict__ v,
                                                     const int* __restrict__ sstart, int cbits, T* __restrict__ part) {
     __shared__ T sh[SB];
@@ -256,13 +265,14 @@ __global__ __launch_bounds__(WG) void k_sorted_slab(int rows, int nslab, const i
     const int q = int(blockIdx.x) % nslab, lb = int(blockIdx.x) / nslab;
     if (lb >= snblk[q]) return;
     const int* blk = blk_all + sboff[q];
-    const int* ptr = ptr_all + size_t(q) * (size_t(rows) + 1);
+    const int* bk = bk_all + sboff[q];
+    const uint16_t* roff = roff_all + size_t(q) * size_t(rows);
     const T* vq = v + sstart[q];
     T* y = part + size_t(q) * size_t(rows);
     const int tid = int(threadIdx.x);
     const uint32_t mask = (1u << cbits) - 1u;
     const int r0 = blk[lb], r1 = blk[lb + 1];
-    const int k0 = ptr[r0], k1 = ptr[r1];
+    const int k0 = bk[lb], k1 = bk[lb + 1];
     if (k1 - k0 > kCap && r1 - r0 == 1) {  // a long row of this slab: strided sums and a workgroup reduction
         T s = 0;
         gather_rounds<WG, 8>(k0, k1, pk, val, vq, ColPacked{mask}, [&](int, uint32_t, T x) { s += x; });
@@ -274,7 +284,9 @@ __global__ __launch_bounds__(WG) void k_sorted_slab(int rows, int nslab, const i
     }
     gather_rounds<WG, 8>(k0, k1, pk, val, vq, ColPacked{mask}, [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
     __syncthreads();
-    block_row_sums<T, WG>(ptr, r0, r1, k0, sh, [&](int r, T s) { y[r] = s; });
+    const int len = k1 - k0;
+    block_row_sums_at<T, WG>([&](int r) { return r < r1 ? int(roff[r]) : len; }, r0, r1, sh,
+                             [&](int r, T s) { y[r] = s; });
 }
 
 // Slab layout, first kernel: workgroup b handles row block b / nslab of slab b % nslab (on XCD b % 8) and writes
@@ -487,8 +499,9 @@ struct Mem {  // device allocations owned by one backend
 //  plain: row blocks over the whole matrix (k_rowblocks).
 //  slab:  for a gathered vector of at least kSlabMinCols entries (half an XCD's L2 in FP32) whose (row, slab)
 //         pairs are mostly occupied (slabs_pay), the columns are split into nslab equal ranges (slab_count: 8 or
-//         16) and the matrix stored slab-major as nslab CSR pieces
-//         with absolute row pointers. Workgroup b works on slab b % nslab, i.e. on XCD b % 8, so each XCD gathers
+//         16) and the matrix stored slab-major as nslab CSR pieces, with absolute row pointers (row blocks) or
+//         block starts and 16-bit in-block row offsets (sorted blocks). Workgroup b works on slab b % nslab, i.e.
+//         on XCD b % 8, so each XCD gathers
 //         from slices of the vector that stay in its own L2; each slab writes partial row sums and a combine
 //         kernel adds them in slab order and applies the epilogue.
 constexpr int kSlabs = kSlabsK;
@@ -498,7 +511,7 @@ template <class T>
 struct DevMat {
     int rows = 0, nblk = 0, maxblk = 0;
     bool slab = false;
-    int *ptr = nullptr, *idx = nullptr, *blk = nullptr;  // slab: ptr holds nslab * (rows + 1) entries
+    int *ptr = nullptr, *idx = nullptr, *blk = nullptr;  // unsorted slab: ptr holds nslab * (rows + 1) entries
     int *snblk = nullptr, *sboff = nullptr;              // slab: block count and block-list offset per slab
     T* val = nullptr;
     T* part = nullptr;  // slab: nslab * rows partial sums
@@ -509,6 +522,8 @@ struct DevMat {
     int cbits = 0, sb = 0;
     int* sstart = nullptr;                               // sorted slab: first column of each slab
     int nslab = 0;                                       // slab: number of column slabs (2, 4, 8 or 16)
+    int* bk = nullptr;           // sorted slab: first nonzero of each block (indexed like blk; ptr is not uploaded)
+    uint16_t* roff = nullptr;    // sorted slab: nslab * rows offsets of each row's first nonzero inside its block
 };
 
 struct HostLayout {
@@ -521,6 +536,10 @@ struct HostLayout {
     int cbits = 0, sb = 0;  // sorted: column bits of the packed words, nonzeros per block (power of two)
     std::vector<int32_t> sstart;
     int nslab = 0;
+    // sorted slab (ptr left empty): first nonzero of each block, indexed like blk, and each row's offset inside its
+    // block per slab (nslab * rows; a block holds at most 2^14 nonzeros, so 16 bits suffice)
+    std::vector<int32_t> bk;
+    std::vector<uint16_t> roff;
 };
 
 // Column slabs of the slab layout: the fewest of 8 and 16 that keep a slab's slice of the gathered vector within
@@ -714,7 +733,24 @@ bool slabs_pay(int rows, int cols, const int32_t* ptr, const int32_t* idx, int n
         L.maxblk = std::max(L.maxblk, L.snblk[size_t(q)]);
         L.blk.insert(L.blk.end(), b.begin(), b.end());
     }
-    if (L.sorted) L.idx.swap(pk), L.perm.swap(perm2);
+    if (L.sorted) {
+        L.idx.swap(pk), L.perm.swap(perm2);
+        // the kernels locate blocks and rows by bk and roff; the per-slab row pointers stay on the host
+        L.bk.resize(L.blk.size()), L.roff.assign(size_t(NS) * size_t(rows), 0);
+        parallel_for(NS, 1, [&](int64_t q) {
+            const int32_t* qptr = L.ptr.data() + size_t(q) * (size_t(rows) + 1);
+            const std::vector<int32_t>& b = bq[size_t(q)];
+            const size_t base = size_t(L.sboff[size_t(q)]);
+            for (size_t lb = 0; lb < b.size(); ++lb) L.bk[base + lb] = qptr[b[lb]];
+            for (size_t lb = 0; lb + 1 < b.size(); ++lb)
+                for (int32_t r = b[lb]; r < b[lb + 1]; ++r) {  // a one-row block (a long row) gives offset 0
+                    const int32_t o = qptr[r] - qptr[b[lb]];
+                    if (o > 65535) throw std::logic_error("sorted slab block beyond 16-bit row offsets");
+                    L.roff[size_t(q) * size_t(rows) + size_t(r)] = uint16_t(o);
+                }
+        });
+        L.ptr.clear(), L.ptr.shrink_to_fit();
+    }
     return L;
 }
 
@@ -744,7 +780,12 @@ DevMat<T> upload_layout(Mem& mem, hipStream_t st, int rows, const HostLayout& L,
     };
     d.ptr = up(L.ptr), d.idx = up(L.idx), d.blk = up(L.blk);
     d.sorted = L.sorted, d.cbits = L.cbits, d.sb = L.sb, d.nslab = L.nslab;
-    if (L.sorted && L.slab) d.sstart = up(L.sstart);
+    if (L.sorted && L.slab) {
+        d.sstart = up(L.sstart), d.bk = up(L.bk);
+        d.roff = mem.alloc<uint16_t>(std::max<size_t>(L.roff.size(), 1));
+        if (!L.roff.empty())
+            HIP_OK(hipMemcpyAsync(d.roff, L.roff.data(), L.roff.size() * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+    }
     d.nch = int(L.chunk.size() / 3), d.nlr = int(L.lrow.size() / 3);
     if (d.nch) d.chunk = up(L.chunk), d.lrow = up(L.lrow), d.lpart = mem.alloc<T>(size_t(d.nch));
     if (L.slab) {
@@ -764,6 +805,7 @@ DevMat<U> share_structure(Mem& mem, hipStream_t st, const DevMat<T>& a, const Ho
     d.ptr = a.ptr, d.idx = a.idx, d.blk = a.blk, d.snblk = a.snblk, d.sboff = a.sboff;
     d.nch = a.nch, d.nlr = a.nlr, d.chunk = a.chunk, d.lrow = a.lrow;
     d.sorted = a.sorted, d.cbits = a.cbits, d.sb = a.sb, d.sstart = a.sstart, d.nslab = a.nslab;
+    d.bk = a.bk, d.roff = a.roff;
     if (a.nch) d.lpart = mem.alloc<U>(size_t(a.nch));
     d.val = upload_values<U>(mem, st, L, val, int64_t(L.idx.size()));
     if (a.slab) d.part = mem.alloc<U>(size_t(a.nslab) * size_t(std::max(a.rows, 1)));
@@ -781,9 +823,9 @@ void launch_sorted(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t s
     if (a.slab) {
         constexpr int WG = stage_wg(SB);
         if (a.maxblk > 0)
-            k_sorted_slab<T, SB, WG><<<a.nslab * a.maxblk, WG, 0, st>>>(a.rows, a.nslab, a.snblk, a.sboff, a.blk,
-                                                                      a.ptr, reinterpret_cast<const uint32_t*>(a.idx),
-                                                                      a.val, v, a.sstart, a.cbits, a.part);
+            k_sorted_slab<T, SB, WG><<<a.nslab * a.maxblk, WG, 0, st>>>(
+                a.rows, a.nslab, a.snblk, a.sboff, a.blk, a.bk, a.roff, reinterpret_cast<const uint32_t*>(a.idx),
+                a.val, v, a.sstart, a.cbits, a.part);
         k_combine_epi<T, Epi><<<grid_for(a.rows), kWG, 0, st>>>(a.rows, a.nslab, a.part, epi);
         return;
     }

@@ -112,15 +112,28 @@ static void run(const Case& c, bool sorted_mode, unsigned seed) {
         std::vector<double> part(size_t(L.nslab) * size_t(c.rows), NAN);
         const int WG = L.sorted ? wg_of(L.sb) : kWG;
         CHECK(int(L.sstart.size()) == L.nslab + 1);
+        if (L.sorted) {  // block starts and 16-bit in-block row offsets replace the per-slab row pointers
+            CHECK(L.ptr.empty() && L.bk.size() == L.blk.size() && L.roff.size() == size_t(L.nslab) * size_t(c.rows));
+        } else {
+            CHECK(L.ptr.size() == size_t(L.nslab) * (size_t(c.rows) + 1));
+        }
         for (int q = 0; q < L.nslab; ++q) {
             const int32_t* blk = L.blk.data() + L.sboff[size_t(q)];
-            const int32_t* qp = L.ptr.data() + size_t(q) * (size_t(c.rows) + 1);
+            const int32_t* bk = L.sorted ? L.bk.data() + L.sboff[size_t(q)] : nullptr;
+            const uint16_t* roff = L.sorted ? L.roff.data() + size_t(q) * size_t(c.rows) : nullptr;
+            const int32_t* qp = L.sorted ? nullptr : L.ptr.data() + size_t(q) * (size_t(c.rows) + 1);
             const int first = L.sorted ? L.sstart[size_t(q)] : 0;
             const int width = L.sorted ? L.sstart[size_t(q) + 1] - L.sstart[size_t(q)] : c.cols;
             CHECK(L.snblk[size_t(q)] <= L.maxblk && blk[0] == 0 && blk[L.snblk[size_t(q)]] == c.rows);
+            if (L.sorted) CHECK(bk[0] >= 0 && bk[L.snblk[size_t(q)]] <= int(L.idx.size()));
             for (int lb = 0; lb < L.snblk[size_t(q)]; ++lb) {
-                const int r0 = blk[lb], r1 = blk[lb + 1], k0 = qp[r0], k1 = qp[r1];
+                const int r0 = blk[lb], r1 = blk[lb + 1];
+                const int k0 = L.sorted ? bk[lb] : qp[r0], k1 = L.sorted ? bk[lb + 1] : qp[r1];
                 CHECK(r1 > r0 && k1 >= k0);
+                // row r's nonzeros in the layout: [rs(r), rs(r + 1)), absolute
+                auto rs = [&](int r) { return L.sorted ? (r < r1 ? k0 + int(roff[r]) : k1) : qp[r]; };
+                if (L.sorted)
+                    for (int r = r0; r < r1; ++r) CHECK(rs(r) <= rs(r + 1) && (r > r0 || roff[r] == 0));
                 if (k1 - k0 > kCap && r1 - r0 == 1) {  // one long row: strided sums
                     double s = 0;
                     for (int k = k0; k < k1; ++k) {
@@ -152,7 +165,7 @@ static void run(const Case& c, bool sorted_mode, unsigned seed) {
                 }
                 for (int r = r0; r < r1; ++r) {
                     double s = 0;
-                    for (int j = qp[r] - k0; j < qp[r + 1] - k0; ++j) {
+                    for (int j = rs(r) - k0; j < rs(r + 1) - k0; ++j) {
                         CHECK(!std::isnan(sh[size_t(j)]));
                         s += sh[size_t(j)];
                     }
