@@ -36,13 +36,17 @@ struct EpiRow {
     T *Ax, *z, *Axs, *zs;
     const T *rl, *ru, *par;
     T sig;
+    struct Pre {
+        T zi, axi, lo, hi, axs, zs;
+    };
     __device__ void load() { sig = par[1]; }
-    __device__ void operator()(int i, T s) const {
-        const T v = z[i] + sig * (2 * s - Ax[i]), q = v / sig;
-        const T lo = rl[i], hi = ru[i];
-        const T zn = q < lo ? v - sig * lo : (q > hi ? v - sig * hi : T(0));
-        Ax[i] = s, z[i] = zn, Axs[i] += s, zs[i] += zn;
+    __device__ Pre pre(int i) const { return Pre{z[i], Ax[i], rl[i], ru[i], Axs[i], zs[i]}; }
+    __device__ void apply(int i, T s, const Pre& p) const {
+        const T v = p.zi + sig * (2 * s - p.axi), q = v / sig;
+        const T zn = q < p.lo ? v - sig * p.lo : (q > p.hi ? v - sig * p.hi : T(0));
+        Ax[i] = s, z[i] = zn, Axs[i] = p.axs + s, zs[i] = p.zs + zn;
     }
+    __device__ void operator()(int i, T s) const { apply(i, s, pre(i)); }
 };
 
 // Column step, gpu/pdhg.cu:193-206: s = (A^T z)_j; ATz = s, ATzs += s; when upd, the next primal step
@@ -53,14 +57,22 @@ struct EpiCol {
     const T *c, *l, *u, *par;
     int upd;
     T tau;
+    struct Pre {
+        T atzs, xj, c, l, u, xs;
+    };
     __device__ void load() { tau = par[0]; }
-    __device__ void operator()(int j, T s) const {
-        ATz[j] = s, ATzs[j] += s;
+    __device__ Pre pre(int j) const {
+        if (upd) return Pre{ATzs[j], x[j], c[j], l[j], u[j], xs[j]};
+        return Pre{ATzs[j], T(0), T(0), T(0), T(0), T(0)};
+    }
+    __device__ void apply(int j, T s, const Pre& p) const {
+        ATz[j] = s, ATzs[j] = p.atzs + s;
         if (upd) {
-            const T xn = clampv(x[j] - tau * (c[j] + s), l[j], u[j]);
-            x[j] = xn, xs[j] += xn;
+            const T xn = clampv(p.xj - tau * (p.c + s), p.l, p.u);
+            x[j] = xn, xs[j] = p.xs + xn;
         }
     }
+    __device__ void operator()(int j, T s) const { apply(j, s, pre(j)); }
 };
 
 // First primal step of a block from the stored A^T z (gpu/pdhg.cu:208-214).

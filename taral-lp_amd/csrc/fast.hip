@@ -29,14 +29,17 @@ struct EpiHRow {
     const int* kc;
     int s;
     T rho, sig, lam;
+    struct Pre {
+        T zi, h, lo, hi, z0;
+    };
     __device__ void load() { sig = par[1], lam = halpern_lambda<T>(kc, s); }
-    __device__ void operator()(int i, T a) const {
-        const T zi = z[i], h = zh[i];
-        const T v = zi + sig * a, q = (v + h) / sig;
-        const T lo = rl[i], hi = ru[i];
-        const T zp = q < lo ? v - sig * lo : (q > hi ? v - sig * hi : -h);
-        z[i] = lam * (rho * zp - (rho - 1) * zi) + (1 - lam) * z0[i];
+    __device__ Pre pre(int i) const { return Pre{z[i], zh[i], rl[i], ru[i], z0[i]}; }
+    __device__ void apply(int i, T a, const Pre& p) const {
+        const T v = p.zi + sig * a, q = (v + p.h) / sig;
+        const T zp = q < p.lo ? v - sig * p.lo : (q > p.hi ? v - sig * p.hi : -p.h);
+        z[i] = lam * (rho * zp - (rho - 1) * p.zi) + (1 - lam) * p.z0;
     }
+    __device__ void operator()(int i, T a) const { apply(i, a, pre(i)); }
 };
 
 // Column kernel epilogue: Halpern blend of the primal (mode 0) or none (mode 1, first step after a restart), then
@@ -48,17 +51,25 @@ struct EpiHCol {
     const int* kc;
     int s, mode;
     T rho, tau, lam;
+    struct Pre {
+        T xj, xpj, x0j, c, l, u;
+    };
     __device__ void load() { tau = par[0], lam = halpern_lambda<T>(kc, s); }
-    __device__ void operator()(int j, T a) const {
-        T xj = x[j];
+    __device__ Pre pre(int j) const {
+        if (mode == 0) return Pre{x[j], xp[j], x0[j], c[j], l[j], u[j]};
+        return Pre{x[j], T(0), T(0), c[j], l[j], u[j]};
+    }
+    __device__ void apply(int j, T a, const Pre& p) const {
+        T xj = p.xj;
         if (mode == 0) {
-            xj = lam * (rho * xp[j] - (rho - 1) * xj) + (1 - lam) * x0[j];
+            xj = lam * (rho * p.xpj - (rho - 1) * xj) + (1 - lam) * p.x0j;
             x[j] = xj;
         }
-        const T xn = clampv(xj - tau * (c[j] + a), l[j], u[j]);
+        const T xn = clampv(xj - tau * (p.c + a), p.l, p.u);
         xp[j] = xn, w[j] = 2 * xn - xj;
         if (ATz) ATz[j] = a;
     }
+    __device__ void operator()(int j, T a) const { apply(j, a, pre(j)); }
 };
 
 __global__ void k_advance(int* kc, int steps) {

@@ -54,10 +54,16 @@ __device__ __forceinline__ T row_prox(T zi, T a, T h, T lo, T hi, T sig) {
     return q < lo ? v - sig * lo : (q > hi ? v - sig * hi : -h);
 }
 
+// Epilogues of the row-block kernels: load() reads the launch's scalars, pre(r) the operands of row r and
+// apply(r, sum, pre) finishes the row; the block kernels call pre before their gathers so that the operands'
+// latency overlaps them, and operator()(r, sum) does both at once.
 template <class T>
 struct EpiStore {  // y = M v
     T* y;
+    struct Pre {};
     __device__ void load() {}
+    __device__ Pre pre(int) const { return {}; }
+    __device__ void apply(int r, T s, const Pre&) const { y[r] = s; }
     __device__ void operator()(int r, T s) const { y[r] = s; }
 };
 
@@ -95,62 +101,6 @@ struct ColPacked {
     uint32_t mask;
     __device__ uint32_t operator()(uint32_t p) const { return p & mask; }
 };
-
-// Row-block SpMV. Workgroups [0, nblk) take the row blocks; a row longer than kCap nonzeros forms a block of its
-// own, which is skipped here: workgroups [nblk, nblk + nch) each take one kCap-nonzero chunk of such a row
-// (chunk c: row, first and end nonzero in chunk[3c..3c+2]) and write its sum to lpart[c]; k_long_rows then adds
-// each long row's chunk sums in chunk order and applies the epilogue (deterministic, no atomics).
-template <class T, class Epi>
-__global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restrict__ blk, const int* __restrict__ ptr,
-                                                   const int* __restrict__ idx, const T* __restrict__ val,
-                                                   const T* __restrict__ v, Epi epi, const int* __restrict__ chunk,
-                                                   T* __restrict__ lpart) {
-    __shared__ T sh[kCap];
-    __shared__ T red[kWG / 64];
-    const int tid = int(threadIdx.x);
-    if (int(blockIdx.x) >= nblk) {  // a chunk of a long row
-        const int c = int(blockIdx.x) - nblk;
-        const int k0 = chunk[3 * c + 1], k1 = chunk[3 * c + 2];
-        T s = 0;
-        gather_rounds<kWG, 8>(k0, k1, idx, val, v, ColPlain{}, [&](int, int, T x) { s += x; });
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if ((tid & 63) == 0) red[tid >> 6] = s;
-        __syncthreads();
-        if (tid == 0) lpart[c] = (red[0] + red[1]) + (red[2] + red[3]);
-        return;
-    }
-    const int b = xcd_block(int(blockIdx.x), nblk);
-    const int r0 = blk[b], r1 = blk[b + 1];
-    const int k0 = ptr[r0], k1 = ptr[r1];
-    if (k1 - k0 > kCap) return;  // a long row: its chunks and k_long_rows handle it
-    epi.load();
-    gather_rounds<kWG, kCap / kWG>(k0, k1, idx, val, v, ColPlain{}, [&](int k, int, T x) { sh[k - k0] = x; });
-    __syncthreads();
-    const int nr = r1 - r0;
-    int lg = 6;  // lanes per row = 2^lg: the widest group with nr groups fitting the workgroup
-    while (lg > 0 && (nr << lg) > kWG) --lg;
-    const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
-    T s = 0;
-    if (row < nr) {
-        const int a = ptr[r0 + row] - k0, e = ptr[r0 + row + 1] - k0;
-        for (int k = a + lane; k < e; k += lpr) s += sh[k];
-    }
-    for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (row < nr && lane == 0) epi(r0 + row, s);
-}
-
-// Long rows: row lrow[3i], chunks [lrow[3i+1], lrow[3i+1] + lrow[3i+2]) of lpart, summed in order.
-template <class T, class Epi>
-__global__ __launch_bounds__(kWG) void k_long_rows(int nlr, const int* __restrict__ lrow, const T* __restrict__ lpart,
-                                                   Epi epi) {
-    epi.load();
-    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < nlr; i += int(gridDim.x * blockDim.x)) {
-        const int c0 = lrow[3 * i + 1], nc = lrow[3 * i + 2];
-        T s = lpart[c0];
-        for (int c = 1; c < nc; ++c) s += lpart[c0 + c];
-        epi(lrow[3 * i], s);
-    }
-}
 
 // Row sums of a staged block (products of rows [r0, r1) in CSR order at sh[off(r) ...], off(r1) = the block's
 // length) by a workgroup of WG threads: blocks of at most WG rows give each row a power-of-two lane group (xor
@@ -195,6 +145,84 @@ template <class T, int WG = kWG, class Out>
 __device__ __forceinline__ void block_row_sums(const int* __restrict__ ptr, int r0, int r1, int k0, const T* sh,
                                                Out&& out) {
     block_row_sums_at<T, WG>([&](int r) { return ptr[r] - k0; }, r0, r1, sh, out);
+}
+
+// A staged block from start to finish: stage() fills sh, then the rows are summed and finished exactly as
+// block_row_sums_at sums them (same rows per thread, same order). When the block has at most WG rows (one lane
+// group per row), each group's row offsets and epilogue operands are loaded before stage(), so their latency
+// overlaps the gathers; larger blocks load them after (four rows per thread ahead would cost the registers of
+// four operand sets: 50 -> 84 VGPRs for the 8,192-nonzero blocks, and 4% on config 2, whose blocks are larger).
+template <class T, int WG, class Off, class Epi, class Stage>
+__device__ __forceinline__ void staged_rows(Off&& off, int r0, int r1, const T* sh, const Epi& epi, Stage&& stage) {
+    const int tid = int(threadIdx.x), nr = r1 - r0;
+    if (nr > WG) {
+        stage();
+        __syncthreads();
+        block_row_sums_at<T, WG>(off, r0, r1, sh, [&](int r, T s) { epi(r, s); });
+        return;
+    }
+    int lg = 6;
+    while (lg > 0 && (nr << lg) > WG) --lg;
+    const int row = tid >> lg, lane = tid & ((1 << lg) - 1), lpr = 1 << lg;
+    int a = 0, e = 0;
+    typename Epi::Pre pp{};
+    if (row < nr) {
+        a = off(r0 + row), e = off(r0 + row + 1);
+        if (lane == 0) pp = epi.pre(r0 + row);
+    }
+    stage();
+    __syncthreads();
+    T s = 0;
+    for (int k = a + lane; k < e; k += lpr) s += sh[k];
+    for (int o = lpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (row < nr && lane == 0) epi.apply(r0 + row, s, pp);
+}
+
+// Row-block SpMV. Workgroups [0, nblk) take the row blocks; a row longer than kCap nonzeros forms a block of its
+// own, which is skipped here: workgroups [nblk, nblk + nch) each take one kCap-nonzero chunk of such a row
+// (chunk c: row, first and end nonzero in chunk[3c..3c+2]) and write its sum to lpart[c]; k_long_rows then adds
+// each long row's chunk sums in chunk order and applies the epilogue (deterministic, no atomics).
+template <class T, class Epi>
+__global__ __launch_bounds__(kWG) void k_rowblocks(int nblk, const int* __restrict__ blk, const int* __restrict__ ptr,
+                                                   const int* __restrict__ idx, const T* __restrict__ val,
+                                                   const T* __restrict__ v, Epi epi, const int* __restrict__ chunk,
+                                                   T* __restrict__ lpart) {
+    __shared__ T sh[kCap];
+    __shared__ T red[kWG / 64];
+    const int tid = int(threadIdx.x);
+    if (int(blockIdx.x) >= nblk) {  // a chunk of a long row
+        const int c = int(blockIdx.x) - nblk;
+        const int k0 = chunk[3 * c + 1], k1 = chunk[3 * c + 2];
+        T s = 0;
+        gather_rounds<kWG, 8>(k0, k1, idx, val, v, ColPlain{}, [&](int, int, T x) { s += x; });
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) lpart[c] = (red[0] + red[1]) + (red[2] + red[3]);
+        return;
+    }
+    const int b = xcd_block(int(blockIdx.x), nblk);
+    const int r0 = blk[b], r1 = blk[b + 1];
+    const int k0 = ptr[r0], k1 = ptr[r1];
+    if (k1 - k0 > kCap) return;  // a long row: its chunks and k_long_rows handle it
+    epi.load();
+    // a block holds at most kWG rows, so each row gets a power-of-two lane group (the widest that fits)
+    staged_rows<T, kWG>([&](int r) { return ptr[r] - k0; }, r0, r1, sh, epi, [&] {
+        gather_rounds<kWG, kCap / kWG>(k0, k1, idx, val, v, ColPlain{}, [&](int k, int, T x) { sh[k - k0] = x; });
+    });
+}
+
+// Long rows: row lrow[3i], chunks [lrow[3i+1], lrow[3i+1] + lrow[3i+2]) of lpart, summed in order.
+template <class T, class Epi>
+__global__ __launch_bounds__(kWG) void k_long_rows(int nlr, const int* __restrict__ lrow, const T* __restrict__ lpart,
+                                                   Epi epi) {
+    epi.load();
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < nlr; i += int(gridDim.x * blockDim.x)) {
+        const int c0 = lrow[3 * i + 1], nc = lrow[3 * i + 2];
+        T s = lpart[c0];
+        for (int c = 1; c < nc; ++c) s += lpart[c0 + c];
+        epi(lrow[3 * i], s);
+    }
 }
 
 // Pairwise sum of N values (N a power of two): ((r0 + r1) + (r2 + r3)) + ...
@@ -243,9 +271,9 @@ __global__ __launch_bounds__(WG) void k_sorted(int nblk, const int* __restrict__
     const int k0 = ptr[r0], k1 = ptr[r1];
     if (k1 - k0 > kCap && r1 - r0 == 1) return;  // a long row: its chunks and k_long_rows handle it
     epi.load();
-    gather_rounds<WG, 8>(k0, k1, pk, val, v, ColPacked{mask}, [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
-    __syncthreads();
-    block_row_sums<T, WG>(ptr, r0, r1, k0, sh, [&](int r, T s) { epi(r, s); });
+    staged_rows<T, WG>([&](int r) { return ptr[r] - k0; }, r0, r1, sh, epi, [&] {
+        gather_rounds<WG, 8>(k0, k1, pk, val, v, ColPacked{mask}, [&](int, uint32_t p, T x) { sh[p >> cbits] = x; });
+    });
 }
 
 // Sorted slab layout: workgroup b takes row block b / nslab of slab b % nslab (columns local to the slab,
