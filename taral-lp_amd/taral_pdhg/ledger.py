@@ -1,9 +1,11 @@
 """Benchmark ledger in the reference's CSV schema (gpu/bench_pdhg.py:23-27), so new results line up with the
 reference's results/gpu_pdhg/*.csv rows.
 
-    python -m taral_pdhg.ledger CASE.mps [CASE.mps ...] --out ledger.csv [--tol 1e-4] [--repeats 3] [--fast]
+    python -m taral_pdhg.ledger CASE [CASE ...] --out ledger.csv [--tol 1e-4] [--repeats 3] [--fast]
 
-Each case is read once, solved ``--repeats`` times on the GPU (a fresh tpdhg_solve each time, like the reference
+A CASE is an MPS file, or `packing:N` / `transport:SxD` for the reference's generated cases (gpu/gen_lp.py, restated
+in gen.py), built in memory under the names its ledgers use (results/gpu_pdhg/synth_*.csv: packing_N,
+transport_SxD); such a case has no MPS parse in its total_s. Each case is read once, solved ``--repeats`` times on the GPU (a fresh tpdhg_solve each time, like the reference
 harness's fresh process per run), rechecked on the original model by the library's C++ recheck (the chk_* columns,
 gpu/vector_check.py's quantities) and compared with HiGHS. The reference harness calls highspy, which this image
 does not have; the HiGHS that scipy bundles (scipy.optimize.linprog, method "highs") gives highs_objective and
@@ -19,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import ffi
+from . import ffi, gen
 
 FIELDS = ['case', 'rows', 'cols', 'nnz', 'device', 'threads', 'precision', 'tol', 'status', 'objective',
           'highs_objective', 'objective_rel_error_vs_highs', 'rel_primal', 'rel_dual', 'rel_gap', 'max_row_violation',
@@ -65,8 +67,19 @@ def highs_objective(model: ffi.Model):
     return sign * res.fun + info.obj_const, wall
 
 
+def load_case(spec: str):
+    """(name, Model) of a CASE argument: an MPS file or a generated packing / transport case."""
+    kind, _, arg = spec.partition(":")
+    if kind == "packing" and arg:
+        return f"packing_{int(arg)}", ffi.Model.from_lp(gen.packing_mps(int(arg), text=False)[0])
+    if kind == "transport" and arg:
+        s, d = (int(v) for v in arg.split("x"))
+        return f"transport_{s}x{d}", ffi.Model.from_lp(gen.transport_mps(s, d)[0])
+    return os.path.splitext(os.path.basename(spec))[0], ffi.Model.from_mps(spec)
+
+
 def case_row(path: str, tol: float, repeats: int, fast: bool, time_limit: float, highs: bool = True) -> dict:
-    model = ffi.Model.from_mps(path)
+    name, model = load_case(path)
     info = model.info()
     opts = dict(tol=tol, time_limit=time_limit)
     if fast:
@@ -79,7 +92,7 @@ def case_row(path: str, tol: float, repeats: int, fast: bool, time_limit: float,
     ref, highs_s = highs_objective(model) if highs else (None, None)
     med = lambda k: statistics.median(r[k] for r in runs)
     return dict(
-        case=os.path.splitext(os.path.basename(path))[0], rows=info.m, cols=info.n, nnz=r0["nnz"], device="gpu",
+        case=name, rows=info.m, cols=info.n, nnz=r0["nnz"], device="gpu",
         threads=0, precision="fp32" if fast else "fp64", tol=tol, status=r0["status"], objective=r0["objective"],
         highs_objective=ref, objective_rel_error_vs_highs=rel_err(r0["objective"], ref),
         rel_primal=r0["rel_primal"], rel_dual=r0["rel_dual"], rel_gap=r0["rel_gap"],
