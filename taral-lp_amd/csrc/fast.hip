@@ -188,8 +188,8 @@ public:
         name_ = device_name();
         HIP_OK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         {
-            const HostLayout LA = make_layout(m_, n_, P.A.ptr.data(), P.A.idx.data(), int(sizeof(T)));
-            const HostLayout LT = make_layout(n_, m_, P.AT.ptr.data(), P.AT.idx.data(), int(sizeof(T)));
+            const HostLayout LA = choose_layout<T>(st_, m_, n_, P.A.ptr.data(), P.A.idx.data(), P.A.val.data());
+            const HostLayout LT = choose_layout<T>(st_, n_, m_, P.AT.ptr.data(), P.AT.idx.data(), P.AT.val.data());
             A_ = upload_layout<T>(mem_, st_, m_, LA, P.A.val.data());
             AT_ = upload_layout<T>(mem_, st_, n_, LT, P.AT.val.data());
             A64_ = values64(A_, LA, P.A);

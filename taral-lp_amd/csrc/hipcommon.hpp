@@ -666,15 +666,18 @@ bool slabs_pay(int rows, int cols, const int32_t* ptr, const int32_t* idx, int n
     return 2 * pairs >= nonempty * nslab;
 }
 
+// one_piece: no column slabs whatever the rule says (the alternative choose_layout times against slabs).
 [[maybe_unused]] HostLayout make_layout(int rows, int cols, const int32_t* ptr, const int32_t* idx,
-                                         int value_bytes = 4) {
+                                         int value_bytes = 4, bool one_piece = false) {
     HostLayout L;
     const int64_t nnz = ptr[rows];
     if (nnz > kMaxNnz)
         throw std::length_error("a matrix of " + std::to_string(nnz) + " nonzeros exceeds the " +
                                 std::to_string(kMaxNnz) + " the kernels index with 32-bit integers");
     const LayoutMode mode = layout_mode();
-    if (mode == LayoutMode::slab)
+    if (one_piece)
+        L.slab = false;
+    else if (mode == LayoutMode::slab)
         L.slab = rows > 0 && cols >= 2 * kSlabs;
     else
         L.slab = cols >= kSlabMinCols && rows > 0 && slabs_pay(rows, cols, ptr, idx, slab_count(cols));
@@ -840,11 +843,6 @@ DevMat<U> share_structure(Mem& mem, hipStream_t st, const DevMat<T>& a, const Ho
     return d;
 }
 
-template <class T>
-DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, int cols, const int32_t* ptr, const int32_t* idx,
-                     const double* val) {
-    return upload_layout<T>(mem, st, rows, make_layout(rows, cols, ptr, idx, int(sizeof(T))), val);
-}
 
 template <class T, int SB, class Epi>
 void launch_sorted(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st) {
@@ -891,6 +889,64 @@ void launch_rows(const DevMat<T>& a, const T* v, const Epi& epi, hipStream_t st)
     if (a.nblk == 0) return;
     k_rowblocks<T, Epi><<<a.nblk + a.nch, kWG, 0, st>>>(a.nblk, a.blk, a.ptr, a.idx, a.val, v, epi, a.chunk, a.lpart);
     if (a.nlr) k_long_rows<T, Epi><<<(a.nlr + kWG - 1) / kWG, kWG, 0, st>>>(a.nlr, a.lrow, a.lpart, epi);
+}
+
+// Median time (ms) of `reps` products y = M v with layout L and values in T, on scratch buffers freed on return
+// (v is zero: the time depends on the indices, not on the values).
+template <class T>
+double time_product(hipStream_t st, int rows, int cols, const HostLayout& L, const double* val, int reps = 7) {
+    Mem tmp;
+    const DevMat<T> d = upload_layout<T>(tmp, st, rows, L, val);
+    T* v = tmp.alloc<T>(size_t(std::max(cols, 1)));
+    T* y = tmp.alloc<T>(size_t(std::max(rows, 1)));
+    HIP_OK(hipMemsetAsync(v, 0, size_t(std::max(cols, 1)) * sizeof(T), st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    std::vector<float> ms;
+    try {
+        for (int r = 0; r < reps + 2; ++r) {  // two warm-up products
+            HIP_OK(hipEventRecord(e0, st));
+            launch_rows(d, v, EpiStore<T>{y}, st);
+            HIP_OK(hipEventRecord(e1, st));
+            HIP_OK(hipEventSynchronize(e1));
+            float t = 0;
+            HIP_OK(hipEventElapsedTime(&t, e0, e1));
+            if (r >= 2) ms.push_back(t);
+        }
+        HIP_OK(hipGetLastError());
+    } catch (...) {
+        (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+    std::sort(ms.begin(), ms.end());
+    return ms[ms.size() / 2];
+}
+
+// Layout of one matrix for the single-LP backends. make_layout's occupancy rule decides between column slabs and
+// one piece; when it picks slabs (automatic mode), the one-piece layout is timed against them (median of 7 products
+// each, the solver's own kernels in T) and taken when it is at least kOnePieceGain times faster, a margin that keeps
+// the choice, and so the summation order, the same from run to run. Measured: the transport LP's A (2,000 rows of
+// 1,000 nonzeros over 1,000,000 columns, 56% of its (row, slab) pairs occupied) 52.3 us per product in 8 slabs
+// against 13.6 us in one piece (FP32; FP64 53.0 / 16.1); config 1's matrices keep their slabs.
+constexpr double kOnePieceGain = 1.25;
+
+template <class T>
+HostLayout choose_layout(hipStream_t st, int rows, int cols, const int32_t* ptr, const int32_t* idx,
+                         const double* val) {
+    HostLayout L = make_layout(rows, cols, ptr, idx, int(sizeof(T)));
+    if (!L.slab || layout_mode() != LayoutMode::automatic) return L;
+    HostLayout one = make_layout(rows, cols, ptr, idx, int(sizeof(T)), true);
+    const double t_slab = time_product<T>(st, rows, cols, L, val), t_one = time_product<T>(st, rows, cols, one, val);
+    return t_one * kOnePieceGain <= t_slab ? std::move(one) : std::move(L);
+}
+
+// A CSR uploaded in the layout choose_layout picks.
+template <class T>
+DevMat<T> upload_csr(Mem& mem, hipStream_t st, int rows, int cols, const int32_t* ptr, const int32_t* idx,
+                     const double* val) {
+    return upload_layout<T>(mem, st, rows, choose_layout<T>(st, rows, cols, ptr, idx, val), val);
 }
 
 [[maybe_unused]] std::string device_name() {

@@ -10,6 +10,9 @@ column slabs with their combine epilogues) are exercised here in two ways:
 * the ledger's 1M- and 2M-column packing rows (synth_packing_tol1e-4.csv:12,17; synth_packing_tol1e-6.csv:12,17)
   on the default layout selection, which lays their matrices out in sorted slabs (A: 1M / 2M gathered columns)
   and sorted blocks, with the halved FP64 blocks;
+* the ledger's transport_1000x1000 row (synth_transport_tol1e-4.csv:12), whose A (2,000 rows of 1,000 nonzeros
+  over 1M columns) the occupancy rule would lay out in slabs and the timed choice (choose_layout) puts in one piece,
+  on that choice and on forced slabs;
 * the fast path (FP32 iterates + FP64 refinement) at the full sizes of the configurations: the 1M x 2M planted LP,
   the block-angular LP, the 512-LP batch and scenarios of the 50k x 100k scenario base. Each result is rechecked
   by an independent original-space KKT computation (tests/kkt_check.py) at the requested 1e-6 and, where the
@@ -35,6 +38,9 @@ LARGE_PACKING = [  # results/gpu_pdhg/synth_packing_tol1e-4.csv:12,17 and synth_
     dict(N=2000000, tol=1e-4, iterations=640, restarts=4, objective=-1374066.47575),
     dict(N=2000000, tol=1e-6, iterations=6848, restarts=9, objective=-1374063.42034),
 ]
+
+
+LARGE_TRANSPORT = dict(S=1000, D=1000, tol=1e-4, iterations=28032, restarts=22, objective=133.882893894)
 
 
 def _ledger_model(case):
@@ -142,3 +148,28 @@ def test_fast_full_size_scenarios(monkeypatch):
     for t, r in zip(th, res):
         print(f"  {r['status']} it={r['iterations']}", flush=True)
         _check_fast(fam.model(t), r)
+
+
+@pytest.fixture(scope="module")
+def transport_1000x1000():
+    c = LARGE_TRANSPORT
+    return ffi.Model.from_lp(gen.transport_mps(c["S"], c["D"])[0])
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("layout", ["default", "slab"])
+def test_ledger_trajectory_large_transport(layout, transport_1000x1000, monkeypatch):
+    """results/gpu_pdhg/synth_transport_tol1e-4.csv:12 on the timed layout choice and on forced slabs."""
+    monkeypatch.delenv("TPDHG_SLABS", raising=False)
+    if layout == "slab":
+        monkeypatch.setenv("TPDHG_LAYOUT", "slab")
+    else:
+        monkeypatch.delenv("TPDHG_LAYOUT", raising=False)
+    c = LARGE_TRANSPORT
+    print(f"ledger transport_1000x1000 tol={c['tol']} layout={layout}", flush=True)
+    r = ffi.solve(transport_1000x1000, tol=c["tol"], precision_bits=64)
+    print(f"  {r['status']} it={r['iterations']} restarts={r['restarts']} obj={r['objective']:.12g} "
+          f"solve={r['solve_s']:.2f}s", flush=True)
+    assert r["status"] == "near_optimal"
+    assert (r["iterations"], r["restarts"]) == (c["iterations"], c["restarts"])
+    assert r["objective"] == pytest.approx(c["objective"], rel=1e-9)
