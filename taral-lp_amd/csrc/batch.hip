@@ -525,14 +525,21 @@ __global__ __launch_bounds__(kWG) void k_bcombine(int k, const double* __restric
         out[i] = base[i] + double(inner[i]);
 }
 
-// Row blocks with a stage of `cap` nonzeros (row_blocks of hipcommon.hpp for cap = kCap).
+// Row blocks with a stage of `cap` nonzeros (row_blocks of hipcommon.hpp for cap = kCap), the row-sum chains
+// bounded for the stage's workgroup (chain_fits).
 std::vector<int32_t> row_blocks_cap(int rows, const int32_t* ptr, int cap) {
+    const int wg = stage_wg(cap);
     std::vector<int32_t> b;
     b.push_back(0);
     int r0 = 0;
     while (r0 < rows) {
         int r = r0;
-        while (r < rows && r - r0 < kWG && int64_t(ptr[r + 1]) - ptr[r0] <= cap) ++r;
+        int64_t lmax = 0;
+        while (r < rows && r - r0 < kWG && int64_t(ptr[r + 1]) - ptr[r0] <= cap) {
+            const int64_t lm = std::max<int64_t>(lmax, int64_t(ptr[r + 1]) - ptr[r]);
+            if (!chain_fits(r - r0 + 1, lm, wg)) break;
+            lmax = lm, ++r;
+        }
         if (r == r0) r = r0 + 1;
         b.push_back(r);
         r0 = r;

@@ -466,13 +466,39 @@ __global__ __launch_bounds__(kWG) void k_from_f64(int n, const double* __restric
 
 // Row blocks: maximal runs of rows holding at most kCap nonzeros and at most kWG rows; a row longer than
 // kCap forms a block of its own.
+// Lanes the row-sum phase (block_row_sums_at) gives each row of a block of nr rows on wg threads: the widest
+// power-of-two group, at most 64, with nr groups fitting the workgroup; one thread per row beyond wg rows.
+constexpr int row_lanes(int nr, int wg) {
+    int lg = 6;
+    while (lg > 0 && (nr << lg) > wg) --lg;
+    return nr > wg ? 1 : 1 << lg;
+}
+
+// Blocks are cut so that no lane of the row-sum phase adds more than kRowChain products in sequence. The lanes per
+// row follow the block's row count, so a long row among many short or empty rows would otherwise be summed by one
+// or two lanes: a 1,000-nonzero row cut into a block with 255 empty rows took one lane through 1,000 dependent LDS
+// reads, and a product of 125 such rows followed by 875 empty ones 35.8 us instead of 3.6 (the transport LPs' A
+// in column slabs, where each supply row fills one slab and is empty in the others, ran 3.8x slower than in one
+// piece for this reason).
+constexpr int kRowChain = 64;
+
+// Whether a block of nr rows whose longest row has lmax nonzeros keeps every lane within kRowChain additions.
+constexpr bool chain_fits(int nr, int64_t lmax, int wg) {
+    return nr <= 1 || (lmax + row_lanes(nr, wg) - 1) / row_lanes(nr, wg) <= kRowChain;
+}
+
 std::vector<int32_t> row_blocks(int rows, const int32_t* ptr) {
     std::vector<int32_t> b;
     b.push_back(0);
     int r0 = 0;
     while (r0 < rows) {
         int r = r0;
-        while (r < rows && r - r0 < kWG && int64_t(ptr[r + 1]) - ptr[r0] <= kCap) ++r;
+        int64_t lmax = 0;
+        while (r < rows && r - r0 < kWG && int64_t(ptr[r + 1]) - ptr[r0] <= kCap) {
+            const int64_t lm = std::max<int64_t>(lmax, int64_t(ptr[r + 1]) - ptr[r]);
+            if (!chain_fits(r - r0 + 1, lm, kWG)) break;
+            lmax = lm, ++r;
+        }
         if (r == r0) r = r0 + 1;  // long row
         b.push_back(r);
         r0 = r;
@@ -481,12 +507,12 @@ std::vector<int32_t> row_blocks(int rows, const int32_t* ptr) {
 }
 
 // Row blocks of the sorted layout: maximal runs of rows with at most sb nonzeros and at most 8 rows per thread of
-// the block's workgroup (stage_wg(sb)), no row longer than long_above inside a run; a row longer than long_above
-// forms a block of its own (long_above <= sb). The row cap bounds the row-sum phase where many rows are empty (a
+// the block's workgroup (stage_wg(sb)), no row longer than long_above inside a run and every lane's row-sum chain
+// within kRowChain; a row longer than long_above forms a block of its own (long_above <= sb). The row cap bounds the row-sum phase where many rows are empty (a
 // block-structured matrix cut into column slabs leaves most rows of a slab empty; without it one workgroup once
 // wrote 280,000 empty rows' sums and a product took 316 instead of 23 us).
 std::vector<int32_t> sorted_blocks(int rows, const int32_t* ptr, int sb, int long_above = kCap) {
-    const int max_rows = 8 * stage_wg(sb);
+    const int wg = stage_wg(sb), max_rows = 8 * wg;
     std::vector<int32_t> b;
     b.push_back(0);
     int r0 = 0;
@@ -495,9 +521,13 @@ std::vector<int32_t> sorted_blocks(int rows, const int32_t* ptr, int sb, int lon
         if (int64_t(ptr[r0 + 1]) - ptr[r0] > long_above) {
             r = r0 + 1;
         } else {
+            int64_t lmax = 0;
             while (r < rows && r - r0 < max_rows && int64_t(ptr[r + 1]) - ptr[r] <= long_above &&
-                   int64_t(ptr[r + 1]) - ptr[r0] <= sb)
-                ++r;
+                   int64_t(ptr[r + 1]) - ptr[r0] <= sb) {
+                const int64_t lm = std::max<int64_t>(lmax, int64_t(ptr[r + 1]) - ptr[r]);
+                if (!chain_fits(r - r0 + 1, lm, wg)) break;
+                lmax = lm, ++r;
+            }
             if (r == r0) r = r0 + 1;
         }
         b.push_back(r);
@@ -927,9 +957,10 @@ double time_product(hipStream_t st, int rows, int cols, const HostLayout& L, con
 // Layout of one matrix for the single-LP backends. make_layout's occupancy rule decides between column slabs and
 // one piece; when it picks slabs (automatic mode), the one-piece layout is timed against them (median of 7 products
 // each, the solver's own kernels in T) and taken when it is at least kOnePieceGain times faster, a margin that keeps
-// the choice, and so the summation order, the same from run to run. Measured: the transport LP's A (2,000 rows of
-// 1,000 nonzeros over 1,000,000 columns, 56% of its (row, slab) pairs occupied) 52.3 us per product in 8 slabs
-// against 13.6 us in one piece (FP32; FP64 53.0 / 16.1); config 1's matrices keep their slabs.
+// the choice, and so the summation order, the same from run to run. A guard against the rule's misjudgements: it
+// first caught the transport LP's A (2,000 rows of 1,000 nonzeros over 1M columns: 52.3 us per product in 8 slabs
+// against 13.6 in one piece), whose cause, unbounded row-sum chains, kRowChain now removes (11.8 us in slabs, which
+// the timing keeps); config 1's matrices keep their slabs.
 constexpr double kOnePieceGain = 1.25;
 
 template <class T>

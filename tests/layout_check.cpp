@@ -32,6 +32,14 @@ struct Case {
 
 static int wg_of(int sb) { return sb / 16 < kWG ? kWG : (sb / 16 > 1024 ? 1024 : sb / 16); }
 
+// Longest row of rows [r0, r1) under row pointers rp (absolute nonzero positions).
+template <class RP>
+static int64_t longest(int r0, int r1, RP&& rp) {
+    int64_t m = 0;
+    for (int r = r0; r < r1; ++r) m = std::max<int64_t>(m, int64_t(rp(r + 1)) - rp(r));
+    return m;
+}
+
 static void run(const Case& c, bool sorted_mode, unsigned seed) {
     std::mt19937 rng(seed);
     std::vector<int32_t> ptr{0}, idx;
@@ -75,6 +83,7 @@ static void run(const Case& c, bool sorted_mode, unsigned seed) {
                 continue;
             }
             CHECK(r1 - r0 <= kWG);
+            CHECK(chain_fits(r1 - r0, longest(r0, r1, [&](int r) { return L.ptr[size_t(r)]; }), kWG));
             for (int r = r0; r < r1; ++r) {
                 double s = 0;
                 for (int k = L.ptr[size_t(r)]; k < L.ptr[size_t(r) + 1]; ++k) {
@@ -92,6 +101,7 @@ static void run(const Case& c, bool sorted_mode, unsigned seed) {
             CHECK(r1 > r0);
             if (k1 - k0 > kCap && r1 - r0 == 1) continue;  // chunks below
             CHECK(k1 - k0 <= L.sb && r1 - r0 <= 8 * WG);
+            CHECK(chain_fits(r1 - r0, longest(r0, r1, [&](int r) { return L.ptr[size_t(r)]; }), WG));
             std::vector<double> sh(size_t(L.sb), NAN);
             for (int k = k0; k < k1; ++k) {
                 const uint32_t p = uint32_t(L.idx[size_t(k)]), pos = p >> L.cbits, col = p & mask;
@@ -144,6 +154,7 @@ static void run(const Case& c, bool sorted_mode, unsigned seed) {
                     part[size_t(q) * size_t(c.rows) + size_t(r0)] = s;
                     continue;
                 }
+                CHECK(chain_fits(r1 - r0, longest(r0, r1, rs), WG));
                 if (!L.sorted) {
                     CHECK(k1 - k0 <= kCap && r1 - r0 <= kWG);
                     for (int r = r0; r < r1; ++r) {
@@ -235,6 +246,9 @@ int main(int argc, char** argv) {
     const Case cases[] = {
         {"plain", 3000, 5000, 10, 0, 0, 1, 0},
         {"plain_long_rows", 5000, 4000, 30, 3, 3900, 1, 0},  // rows of ~3,900 nonzeros: kCap-nonzero chunks
+        // rows of 1,000 nonzeros between runs of 50 empty rows: blocks cut to bound the row-sum chains
+        {"medium_rows_and_empty_runs", 4000, 20000, 10, 400, 1000, 1, 0, 50},
+        {"slab8_medium_rows_and_empty_runs", 20000, 700000, 10, 400, 1000, 1, 8, 50},
         {"one_row", 1, 10, 3, 0, 0, 1, 0},
         {"slab8", 40000, 600000, 20, 0, 0, 1, 8},
         {"slab8_long_rows", 20000, 700000, 20, 2, 30000, 1, 8},

@@ -11,8 +11,8 @@ column slabs with their combine epilogues) are exercised here in two ways:
   on the default layout selection, which lays their matrices out in sorted slabs (A: 1M / 2M gathered columns)
   and sorted blocks, with the halved FP64 blocks;
 * the ledger's transport_1000x1000 row (synth_transport_tol1e-4.csv:12), whose A (2,000 rows of 1,000 nonzeros
-  over 1M columns) the occupancy rule would lay out in slabs and the timed choice (choose_layout) puts in one piece,
-  on that choice and on forced slabs;
+  over 1M columns; each supply row fills one column slab and is empty in the others) goes through the timed choice
+  between slabs and one piece (choose_layout), on that choice and on forced slabs;
 * the fast path (FP32 iterates + FP64 refinement) at the full sizes of the configurations: the 1M x 2M planted LP,
   the block-angular LP, the 512-LP batch and scenarios of the 50k x 100k scenario base. Each result is rechecked
   by an independent original-space KKT computation (tests/kkt_check.py) at the requested 1e-6 and, where the
