@@ -535,8 +535,8 @@ def main():
     ap.add_argument("--reflection", type=float, default=None,
                     help="fast path: Halpern reflection coefficient in [1, 2) (default: the library's)")
     ap.add_argument("--omega-rule", type=int, default=None,
-                    help="fast path: primal-weight guard (0 reference update, 1 bounded, 2 balance, 3 both; "
-                         "default: the library's, 3)")
+                    help="fast path: primal-weight guard (0 reference update, 1 bounded, 2 balance, 3 both, 4 both "
+                         "with the bound anchored at the first restart's weight; default: the library's, 3)")
     ap.add_argument("--restart-metric", type=int, default=None,
                     help="fast path: what the restart tests measure (1 the fixed-point residual, 0 the weighted KKT "
                          "error; default: the library's, 1)")

@@ -176,7 +176,8 @@ typedef struct {
     double progress_interval; /* > 0: one progress line on stderr every this many seconds of solving */
     int32_t omega_rule; /* fast path and batches: primal-weight guard after each restart. 0: the reference's
                            smoothed update only; 1: kept within two decades of the starting weight; 2: residual
-                           balancing (no move against a 3x primal/gap imbalance); 3 (default): both. The
+                           balancing (no move against a 3x primal/gap imbalance); 3 (default): both; 4: both, the
+                           two decades counted from the weight the first restart's update gives. The
                            reference scheme (halpern = 0) always uses the reference update */
     int32_t reuse_norm; /* 1: a session or batch runs the power iteration on its first solve only and reuses ||A_s||
                            afterwards (A_s never changes within one); 0 (default): every solve runs it, as each

@@ -407,7 +407,7 @@ int check_options(const tpdhg::Options& o) {
     if (o.refine && !o.halpern) return fail(TPDHG_ERR_ARG, "refine runs on the Halpern path: set halpern = 1");
     if (o.halpern && !(o.rho >= 1 && o.rho < 2)) return fail(TPDHG_ERR_ARG, "reflection must be in [1, 2)");
     if (!(o.omega0 >= 0) || !std::isfinite(o.omega0)) return fail(TPDHG_ERR_ARG, "primal_weight must be finite and >= 0");
-    if (o.omega_rule < 0 || o.omega_rule > 3) return fail(TPDHG_ERR_ARG, "omega_rule must be 0, 1, 2 or 3");
+    if (o.omega_rule < 0 || o.omega_rule > 4) return fail(TPDHG_ERR_ARG, "omega_rule must be 0, 1, 2, 3 or 4");
     if (o.restart_metric < 0 || o.restart_metric > 1) return fail(TPDHG_ERR_ARG, "restart_metric must be 0 or 1");
     if (o.cpu_threads < -1 || o.cpu_threads > 4096) return fail(TPDHG_ERR_ARG, "cpu_threads must be -1 (GPU) or 0..4096");
     if (o.cpu_threads >= 0 && o.halpern)

@@ -33,15 +33,19 @@ bool progress_due(const Options& o, Clock::time_point& last) {
 //   2  residual balancing: when the relative primal residual exceeds three times the larger of the dual residual
 //      and the gap, the weight may not fall (a small weight makes the primal step large and the dual step small,
 //      which leaves the primal residual behind); in the opposite case it may not rise;
-//   3  both.
+//   3  both;
+//   4  both, with the bound's anchor w_start moved to the weight the first restart's update gives (the callers
+//      move it): ||c||/||b|| can sit decades from the weight an LP wants (the transport LP 1000 x 1000: 42.4
+//      against the 0.07 the reference update reaches), and the bound around it then holds the weight on its edge
+//      (that LP: 112,448 iterations under rule 3, 28,032 under rule 4).
 double next_omega(double w, double dx, double dz, double p, double dg, double w_start, int rule) {
     double nw = w;
     if (dx > 1e-10 && dz > 1e-10) nw = std::exp(0.5 * std::log(dz / dx) + 0.5 * std::log(w));
-    if (rule == 2 || rule == 3) {
+    if (rule == 2 || rule == 3 || rule == 4) {
         if (p > 3 * dg) nw = std::max(nw, w);
         else if (dg > 3 * p) nw = std::min(nw, w);
     }
-    if (rule == 1 || rule == 3) nw = std::min(std::max(nw, w_start * 1e-2), w_start * 1e2);
+    if (rule == 1 || rule == 3 || rule == 4) nw = std::min(std::max(nw, w_start * 1e-2), w_start * 1e2);
     return nw;
 }
 
@@ -180,7 +184,8 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
     cs = std::sqrt(cs), bs = std::sqrt(bs);
     double w = cs > 1e-10 && bs > 1e-10 ? cs / bs : 1.0;
     if (omega0 > 0) w = omega0;
-    const double w_start = w;
+    double w_start = w;  // the weight guard's anchor (rule 4 moves it to the first restart's weight)
+    bool anchored = false;
     b.set_base(x0, z0);
     // The restart tests' measure of the current point (restart_metric): the fixed-point residual of the last
     // evaluation, ||u - T u||_P = sqrt(||dx||^2 / tau + ||dz||^2 / sigma + 2 dz' A dx) with tau = eta / w and
@@ -256,6 +261,7 @@ Outcome run_fast(FastDevice& b, const Scaled& P, const Options& o, double time_b
         if (restart) {
             const double dx = std::sqrt(kk[DX2]), dz = std::sqrt(kk[DZ2]);
             w = next_omega(w, dx, dz, rc.p, std::max(rc.d, rc.g), w_start, o.omega_rule);
+            if (o.omega_rule == 4 && !anchored) w_start = w, anchored = true;
             if (o.refine && r.refinements < o.max_refinements) {
                 recentre();
                 r.refinements++;
@@ -428,7 +434,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
         w[size_t(i)] = cs > 1e-10 && bs > 1e-10 ? cs / bs : 1.0;
         if (size_t(i) < omega0.size() && omega0[size_t(i)] > 0) w[size_t(i)] = omega0[size_t(i)];
     }
-    const std::vector<double> w_start = w;
+    std::vector<double> w_start = w;  // per instance; rule 4 moves it to the first restart's weight as run_fast
     auto push_steps = [&]() {
         for (int i = 0; i < B; ++i) tau[size_t(i)] = eta[size_t(i)] / w[size_t(i)], sig[size_t(i)] = eta[size_t(i)] * w[size_t(i)];
         b.set_steps(tau, sig);
@@ -506,6 +512,7 @@ std::vector<Outcome> run_batch(BatchDevice& b, const std::vector<const Scaled*>&
                 const double dx = std::sqrt(k[DX2]), dz = std::sqrt(k[DZ2]);
                 w[size_t(i)] = next_omega(w[size_t(i)], dx, dz, rc.p, std::max(rc.d, rc.g), w_start[size_t(i)],
                                           o.omega_rule);
+                if (o.omega_rule == 4 && q.restarts == 0) w_start[size_t(i)] = w[size_t(i)];
                 if (o.refine && q.refinements < o.max_refinements) rs_move[size_t(i)] = 1, any_move = true, q.refinements++;
                 rs_any[size_t(i)] = 1, any_rs = true, q.restarts++;
             }

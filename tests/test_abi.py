@@ -136,6 +136,20 @@ def test_restart_metric_option():
     assert e.value.code == ffi.ERR_NO_DEVICE
 
 
+def test_omega_rule_option():
+    """Weight guards 0-4 are accepted (4: the bound anchored at the first restart's weight), others refused on the
+    host before any device work."""
+    from taral_pdhg import gen
+    md = ffi.Model.from_lp(gen.planted_lp(20, 40, 3, seed=0))
+    for bad in (-1, 5):
+        with pytest.raises(ffi.TpdhgError) as e:
+            ffi.solve(md, halpern=1, omega_rule=bad)
+        assert e.value.code == ffi.ERR_ARG and "omega_rule" in str(e.value)
+    with pytest.raises(ffi.TpdhgError) as e:  # a valid setting reaches the device check
+        ffi.solve(md, halpern=1, omega_rule=4)
+    assert e.value.code == ffi.ERR_NO_DEVICE
+
+
 def test_batch_create_validates_before_the_device():
     from taral_pdhg import gen
     md = ffi.Model.from_lp(gen.planted_lp(20, 40, 3, seed=0))

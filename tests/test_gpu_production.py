@@ -173,3 +173,18 @@ def test_ledger_trajectory_large_transport(layout, transport_1000x1000, monkeypa
     assert r["status"] == "near_optimal"
     assert (r["iterations"], r["restarts"]) == (c["iterations"], c["restarts"])
     assert r["objective"] == pytest.approx(c["objective"], rel=1e-9)
+
+
+@pytest.mark.timeout(300)
+def test_fast_transport_weight_rule_4():
+    """omega_rule 4 (the weight guard's two decades counted from the first restart's weight) on the transport LP
+    whose weight the default guard holds on its lower bound: a 1e-6 solution, rechecked by the independent KKT
+    computation, in far fewer iterations than under rule 3 (measured: 28,032 against 112,448)."""
+    c = LARGE_TRANSPORT
+    md = gen.transport_mps(c["S"], c["D"])[0]
+    print("fast transport_1000x1000, omega_rule 4", flush=True)
+    r = ffi.solve(ffi.Model.from_lp(md), omega_rule=4, **FAST)
+    print(f"  {r['status']} it={r['iterations']} solve={r['solve_s']:.2f}s obj={r['objective']:.10g}", flush=True)
+    _check_fast(md, r)
+    assert r["iterations"] < 60000
+    assert r["objective"] == pytest.approx(133.89572585, rel=1e-5)  # HiGHS, synth_transport_tol1e-6.csv:12
