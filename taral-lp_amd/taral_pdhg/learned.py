@@ -10,7 +10,7 @@ c (gen.ScenarioFamily). The warm start maps theta to a start point (x0, y0) and 
     weight from theta / sigma;
   * the prediction is decoded to x0 (clipped to the bounds), y0 and (optionally) omega0, and handed to the batched
     solver with the cold-start fallback on: an instance whose predicted point has a worse relative KKT than the
-    cold start starts cold, so plain PDHG both judges and backs up the network.
+    cold start starts cold, so plain PDHG both checks and backs up the network.
 """
 from __future__ import annotations
 

@@ -1,15 +1,15 @@
-"""Writes tests/golden/crossover/: this build's PDHG answer files for two small LPs and what the reference's
+"""Writes tests/golden/crossover/: the library's PDHG answer files for two small LPs and what the reference's
 crossover consumer makes of them. Run in the container only (it needs the reference build oracle/_ref/taral, made
 by `make -C oracle ref` from the reference's own sources):
 
     python tests/golden/make_crossover_fixture.py
 
-For each LP the script writes the MPS text (gpu/gen_lp.py-equivalent generator), solves it with this build's `pdhg`
+For each LP the script writes the MPS text (gpu/gen_lp.py-equivalent generator), solves it with the library's `pdhg`
 CLI (CPU backend, reference scheme, tol 1e-6) writing `.sol` (x, %.17g per column) and `.dual` (y = Dr z per row,
 gpu/pdhg.cu:898-909), then runs the reference engine `taral MODEL.mps --method dual` cold and with
 `--warm-sol X.sol --warm-dual X.dual` (src/main.cpp:661-683 -> src/crossover.cpp basis_from_point) and records
 status, objective, iterations and the crossover line. The .sol/.dual files are committed beside the record (the MPS
-text is not: the generator rewrites it) so the CPU test can check that this build still writes them bit for bit.
+text is not: the generator rewrites it) so the CPU test can check that the library still writes them bit for bit.
 """
 import json
 import os

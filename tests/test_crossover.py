@@ -1,6 +1,6 @@
 """The `.sol` / `.dual` wire format into the reference's crossover (src/main.cpp:661-683, src/crossover.cpp:15-60).
 
-tests/golden/make_crossover_fixture.py (container only) fed this build's answer files for two LPs to the reference
+tests/golden/make_crossover_fixture.py (container only) fed the library's answer files for two LPs to the reference
 engine `taral --method dual --warm-sol --warm-dual` and recorded the outcome in tests/golden/crossover/. Here (CPU
 only): the build still writes those answer files bit for bit (CPU backend, deterministic), the reference read them
 and finished optimal from the basis they identify in far fewer dual simplex iterations than from a cold start, and

@@ -3,7 +3,7 @@
 A reduced scenario family (base: the planted generator at m=1,000, n=2,000, 6 nonzeros per column; 32 factors,
 each rescaling a fixed 0.15% of b and of c by exp(theta), theta ~ N(0, 0.1^2)) is solved cold by the batched
 fast path to 1e-6 for 2,048 training scenarios; the network (32 -> 512 -> 512 -> 256 bases + log primal weight,
-bf16 MFMA kernels) is trained on the GPU and judged on 128 held-out scenarios against two references without it:
+bf16 MFMA kernels) is trained on the GPU and evaluated on 128 held-out scenarios against two references without it:
 the cold start and the training-mean solution as a fixed warm start. Asserted: the network's predicted points are
 closer to the held-out solutions than the training mean and closer than the untrained network's; their relative KKT
 beats the cold start's for most scenarios; every warm-started solve reaches 1e-6 (checked independently); and the
