@@ -169,6 +169,7 @@ int main(int argc, char** argv) {
         else if (a == "--halpern") o.halpern = true;
         else if (a == "--reflection") o.rho = std::atof(next());
         else if (a == "--restart-metric") o.restart_metric = std::atoi(next());
+        else if (a == "--omega-rule") o.omega_rule = std::atoi(next());
         else if (a == "--fast") o.halpern = true, o.refine = true, o.precision_bits = 32;
         else if (a == "--max-refinements") o.max_refinements = std::atoi(next());
         else if (a == "--profile-kernels") o.profile_kernels = true;
@@ -178,12 +179,16 @@ int main(int argc, char** argv) {
         else if (a == "--warm-dual") warm_dual = next();
         else model = argv[i];
     }
+    if (o.restart_metric < 0 || o.restart_metric > 1 || o.omega_rule < 0 || o.omega_rule > 4) {
+        std::fprintf(stderr, "--restart-metric must be 0 or 1 and --omega-rule 0, 1, 2, 3 or 4\n");
+        return 2;
+    }
     if (batch_list && !model && device == "gpu") return run_batch_list(batch_list, o, json, do_recheck);
     if (!model || (device != "gpu" && device != "cpu")) {
         std::fprintf(stderr,
                      "usage: pdhg MODEL.mps --device gpu|cpu [--threads N] [--tol 1e-4] [--time-limit S] "
                      "[--max-iter N] [--json OUT.json] [--sol OUT.sol] [--dual OUT.dual] [--fp32] [--eval-freq 64] "
-                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--restart-metric 1] [--refine] "
+                     "[--row-rel 0] [--no-graph] [--fast] [--halpern] [--reflection 1.9] [--restart-metric 1] [--omega-rule 3] [--refine] "
                      "[--max-refinements 8] "
                      "[--profile-kernels] [--warm-sol X.sol] [--warm-dual X.dual] [--recheck]\n"
                      "       pdhg --batch LIST.txt [--json OUT.jsonl] [solver flags]\n");

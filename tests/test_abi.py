@@ -79,6 +79,11 @@ def test_cli_parse_error_and_usage(tmp_path):
     r = subprocess.run([ffi.CLI_PATH, os.path.join(MPS, "gate_continuous_lp.mps"), "--device", "cpu", "--fast",
                         "--json", str(out)], capture_output=True, text=True)
     assert r.returncode == 2 and "GPU only" in r.stderr and not out.exists()
+    # fast-path settings out of range: refused before the model is read or a device touched
+    for flag, bad in (("--omega-rule", "5"), ("--restart-metric", "2")):
+        r = subprocess.run([ffi.CLI_PATH, os.path.join(MPS, "gate_continuous_lp.mps"), "--device", "gpu", "--fast",
+                            flag, bad, "--json", str(out)], capture_output=True, text=True)
+        assert r.returncode == 2 and flag in r.stderr and not out.exists()
 
 
 def test_model_from_csc_validates():
